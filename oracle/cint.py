@@ -1,0 +1,125 @@
+"""ctypes wrapper of liboracle.so (the C restatement in oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        basis = [ctypes.c_int, ip, ip, ip, ip, dp, dp, dp]
+        L.or_boys.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp]
+        L.or_boys.restype = None
+        L.or_nao.argtypes = basis
+        L.or_nao.restype = ctypes.c_int
+        L.or_eri_quartet.argtypes = basis + [ctypes.c_int] * 4 + [dp]
+        L.or_eri_quartet.restype = ctypes.c_int
+        L.or_schwarz.argtypes = basis + [dp]
+        L.or_schwarz.restype = ctypes.c_int
+        L.or_eri_dense.argtypes = basis + [ctypes.c_double, dp, dp]
+        L.or_eri_dense.restype = ctypes.c_longlong
+        L.or_jk.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
+        L.or_jk.restype = None
+        L.or_one_electron.argtypes = basis + [ctypes.c_int, dp, dp, dp, dp, dp]
+        L.or_one_electron.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _basis_args(mol):
+    t = mol.shell_table()
+    keep = [np.ascontiguousarray(t[k]) for k in ("l", "nprim", "prim_off", "spher")]
+    keepd = [np.ascontiguousarray(t[k], dtype=float) for k in ("center", "exps", "cc")]
+    args = [t["nshell"]] + [_i(a) for a in keep] + [_d(a) for a in keepd]
+    return args, (keep, keepd), t
+
+
+def boys(T, mmax):
+    T = np.ascontiguousarray(T, dtype=float).reshape(-1)
+    out = np.zeros((T.size, mmax + 1))
+    lib().or_boys(T.size, _d(T), mmax, _d(out))
+    return out
+
+
+def eri_quartet(mol, a, b, c, d):
+    args, keep, t = _basis_args(mol)
+    nf = t["nfun"]
+    out = np.zeros((nf[a], nf[b], nf[c], nf[d]))
+    rc = lib().or_eri_quartet(*args, a, b, c, d, _d(out))
+    if rc != 0:
+        raise RuntimeError("or_eri_quartet failed: %d" % rc)
+    return out
+
+
+def schwarz(mol):
+    args, keep, t = _basis_args(mol)
+    ns = t["nshell"]
+    out = np.zeros((ns, ns))
+    rc = lib().or_schwarz(*args, _d(out))
+    if rc != 0:
+        raise RuntimeError("or_schwarz failed")
+    return out
+
+
+def eri_dense(mol, thresh=1e-8, bounds=None):
+    """Dense screened (ab|cd) tensor in the final basis; returns (eri, nquartets)."""
+    args, keep, t = _basis_args(mol)
+    if bounds is None:
+        bounds = schwarz(mol)
+    bounds = np.ascontiguousarray(bounds, dtype=float)
+    N = t["nao"]
+    out = np.zeros((N, N, N, N))
+    nq = lib().or_eri_dense(*args, float(thresh), _d(bounds), _d(out))
+    if nq < 0:
+        raise RuntimeError("or_eri_dense failed")
+    return out, int(nq)
+
+
+def jk(eri, Dt, D):
+    """J = einsum('cd,abcd->ab', Dt); K = einsum('cb,abcd->ad', -D)."""
+    N = Dt.shape[0]
+    eri = np.ascontiguousarray(eri, dtype=float)
+    Dt = np.ascontiguousarray(Dt, dtype=float)
+    D = np.ascontiguousarray(D, dtype=float)
+    J = np.zeros((N, N)); K = np.zeros((N, N))
+    lib().or_jk(N, _d(eri), _d(Dt), _d(D), _d(J), _d(K))
+    return J, K
+
+
+def one_electron(mol):
+    """(S, T, V) in the final basis."""
+    args, keep, t = _basis_args(mol)
+    N = t["nao"]
+    S = np.zeros((N, N)); T = np.zeros((N, N)); V = np.zeros((N, N))
+    xyz = np.ascontiguousarray(t["atom_xyz"]); z = np.ascontiguousarray(t["atom_z"])
+    rc = lib().or_one_electron(*args, t["natom"], _d(xyz), _d(z), _d(S), _d(T), _d(V))
+    if rc != 0:
+        raise RuntimeError("or_one_electron failed")
+    return S, T, V

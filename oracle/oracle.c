@@ -1,0 +1,681 @@
+/*
+ * oracle.c -- CPU restatement of the reference integral path (TEST
+ * INFRASTRUCTURE ONLY).
+ *
+ * This file is the parity oracle and the CPU baseline.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product path (pychem_amd) never does.
+ *
+ * It restates, in plain C, the algorithm of dlc62/pychem's `_c_ints`
+ * extension and the Python drivers that schedule it:
+ *   - shell-pair quantities        Methods/c_ints/shellpair_quantities.c:5-38
+ *   - [00|00]^(m) fundamentals     Methods/c_ints/two_electron_fundamentals.c:6-95
+ *     (2*theta^2)^(m+1/2) scaling, P-Q vector; Boys F_m(T) is computed here
+ *     from its series / asymptotic form because the reference's
+ *     interpolation table (interpolation_table.c) is absent from the checkout
+ *   - two-electron OS VRR          Methods/c_ints/two_electron_vrr.c:4-141
+ *     scheduled ket-first then bra as in Methods/integrals.py:73-148,514-517
+ *   - contraction                  Methods/c_ints/two_electron_contract.c:3-57
+ *   - HRR (ket then bra, goofy)    Methods/c_ints/two_electron_hrr.c:4-94,
+ *                                  Methods/integrals.py:531-536,615-635
+ *   - normalisation + spherical    Methods/integrals.py:541-547,
+ *                                  Util/structures.py:843-856
+ *   - bra/ket swap by Ltot         Methods/integrals.py:432-435,552-555
+ *   - one-electron S, T, V         Methods/c_ints/one_electron_*.c,
+ *                                  Methods/integrals.py:220-370
+ *   - Schwarz-screened dense ERIs  Methods/hartree_fock.py:241-325
+ *   - J/K einsums                  Methods/hartree_fock.py:345-347
+ *
+ * Conventions (Appendix A of the reference's behaviour): Cartesian component
+ * order lx desc, then ly desc (Methods/c_ints/angmom_index.c:3-15); each
+ * primitive scaled by (2a)^((l+1.5)/2), each Cartesian component by
+ * (G(lx+1/2)G(ly+1/2)G(lz+1/2))^-1/2; contracted functions not renormalised;
+ * chemists' notation (ab|cd).
+ *
+ * Build: see oracle/Makefile (gcc -O3 -fopenmp -shared).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+
+#define OR_MAXL 2                 /* highest shell angular momentum */
+#define OR_MAXLT (4 * OR_MAXL)    /* highest total L of a quartet */
+#define OR_NLEV 35                /* cartesian components of levels 0..4 */
+#define OR_MAXK 12                /* primitives per shell */
+
+static inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+static inline int cum(int l) { return l * (l + 1) * (l + 2) / 6; }
+/* index of (lx,ly,lz) within its level: Methods/c_ints/angmom_index.c:3-15 */
+static inline int cidx(int ly, int lz) { return (ly + lz) * (ly + lz + 1) / 2 + lz; }
+
+/* component tables for levels 0..OR_MAXLT, filled once */
+static int comp_l[200][3];          /* flat index -> (lx,ly,lz) */
+static int tables_ready = 0;
+static void init_tables(void)
+{
+    if (tables_ready) return;
+    for (int l = 0; l <= OR_MAXLT; l++) {
+        int k = cum(l);
+        for (int lx = l; lx >= 0; lx--)
+            for (int ly = l - lx; ly >= 0; ly--) {
+                int lz = l - lx - ly;
+                comp_l[k][0] = lx; comp_l[k][1] = ly; comp_l[k][2] = lz;
+                k++;
+            }
+    }
+    tables_ready = 1;
+}
+static inline int flat(int lx, int ly, int lz) { return cum(lx + ly + lz) + cidx(ly, lz); }
+
+/* ------------------------------------------------------------------------ */
+/* Boys function F_m(T), m = 0..mmax.                                        */
+/* Series F_m(T) = e^-T sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)) for the    */
+/* top m, downward recursion F_m = (2T F_{m+1} + e^-T)/(2m+1) below it; for  */
+/* large T, F_0 = sqrt(pi/T)/2 erf(sqrt T) and upward recursion.             */
+/* ------------------------------------------------------------------------ */
+void or_boys_one(double T, int mmax, double *F)
+{
+    if (T < 35.0) {
+        long double t = T, term = 1.0L / (2 * mmax + 1), sum = term;
+        for (int k = 1; k < 400; k++) {
+            term *= 2.0L * t / (2 * mmax + 2 * k + 1);
+            sum += term;
+            if (term < 1e-22L * sum) break;
+        }
+        long double et = expl(-t);
+        long double f = et * sum;
+        F[mmax] = (double)f;
+        for (int m = mmax - 1; m >= 0; m--) {
+            f = (2.0L * t * f + et) / (2 * m + 1);
+            F[m] = (double)f;
+        }
+    } else {
+        long double t = T, et = expl(-t);
+        long double f = 0.5L * sqrtl(3.14159265358979323846264338327950288L / t) * erfl(sqrtl(t));
+        F[0] = (double)f;
+        for (int m = 0; m < mmax; m++) {
+            f = ((2 * m + 1) * f - et) / (2.0L * t);
+            F[m + 1] = (double)f;
+        }
+    }
+}
+
+void or_boys(int n, const double *T, int mmax, double *out)
+{
+    for (int i = 0; i < n; i++) or_boys_one(T[i], mmax, out + (size_t)i * (mmax + 1));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Basis description passed from Python (flat arrays).                       */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int nshell;
+    const int *l, *nprim, *prim_off, *spher;   /* per shell */
+    const double *center;                       /* 3 per shell, bohr */
+    const double *exps, *cc;                    /* per primitive; cc already scaled */
+    int *ao_off;                                /* final-basis offset per shell */
+    int *nfun;                                  /* final-basis functions per shell */
+    int nao;
+} basis_t;
+
+static void basis_setup(basis_t *b)
+{
+    int off = 0;
+    for (int s = 0; s < b->nshell; s++) {
+        b->ao_off[s] = off;
+        b->nfun[s] = b->spher[s] ? 2 * b->l[s] + 1 : ncart(b->l[s]);
+        off += b->nfun[s];
+    }
+    b->nao = off;
+}
+
+/* Per-Cartesian-component scaling (G(lx+1/2)G(ly+1/2)G(lz+1/2))^-1/2,
+   Util/structures.py:850-856. */
+static double comp_norm(int lx, int ly, int lz)
+{
+    return 1.0 / sqrt(tgamma(lx + 0.5) * tgamma(ly + 0.5) * tgamma(lz + 0.5));
+}
+
+/* Cartesian -> spherical rows for d, Data/transform_basis.py:8-12. */
+static void spher_rows(int l, double *T /* nsph x ncart */)
+{
+    int nc = ncart(l);
+    if (l < 2) {
+        for (int i = 0; i < nc * nc; i++) T[i] = 0.0;
+        for (int i = 0; i < nc; i++) T[i * nc + i] = 1.0;
+        return;
+    }
+    /* l == 2: components xx, xy, xz, yy, yz, zz */
+    static const double d[5][6] = {
+        {0.86602540378443864676, 0, 0, -0.86602540378443864676, 0, 0},
+        {0, 1, 0, 0, 0, 0},
+        {0, 0, 1, 0, 0, 0},
+        {0, 0, 0, 0, 1, 0},
+        {-0.5, 0, 0, -0.5, 0, 1}};
+    memcpy(T, d, sizeof(d));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Two-electron quartet (HGP), final-basis block [nA][nB][nC][nD].           */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int s1, s2;            /* shells: s1 is the centre the VRR builds on     */
+    int swapped;           /* 1 if s1 is the pair's second shell ("goofy")   */
+    int l1, l2;
+    double X1[3], X2[3];
+} pair_view;
+
+static void make_view(const basis_t *b, int sa, int sb, pair_view *v)
+{
+    /* Goofy: if lB > lA the recurrences run on B (Methods/integrals.py:79-85) */
+    if (b->l[sb] > b->l[sa]) { v->s1 = sb; v->s2 = sa; v->swapped = 1; }
+    else { v->s1 = sa; v->s2 = sb; v->swapped = 0; }
+    v->l1 = b->l[v->s1]; v->l2 = b->l[v->s2];
+    for (int i = 0; i < 3; i++) { v->X1[i] = b->center[3 * v->s1 + i]; v->X2[i] = b->center[3 * v->s2 + i]; }
+}
+
+/* Cartesian contracted, normalised block in view order [c1][c2][c3][c4]. */
+static void eri_cart_view(const basis_t *b, const pair_view *bra, const pair_view *ket, double *out)
+{
+    static const double PI = 3.14159265358979323846;
+    const int la = bra->l1, lb = bra->l2, lc = ket->l1, ld = ket->l2;
+    const int emax = la + lb, fmax = lc + ld, L = emax + fmax;
+    double AB[3], CD[3];
+    for (int i = 0; i < 3; i++) { AB[i] = bra->X1[i] - bra->X2[i]; CD[i] = ket->X1[i] - ket->X2[i]; }
+    double ab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+    double cd2 = CD[0] * CD[0] + CD[1] * CD[1] + CD[2] * CD[2];
+
+    static __thread double vrr[OR_NLEV][OR_NLEV][OR_MAXLT + 1];
+    static __thread double acc[OR_NLEV][OR_NLEV];
+    memset(acc, 0, sizeof(acc));
+
+    const int n1 = b->nprim[bra->s1], n2 = b->nprim[bra->s2];
+    const int n3 = b->nprim[ket->s1], n4 = b->nprim[ket->s2];
+    const double *e1 = b->exps + b->prim_off[bra->s1], *e2 = b->exps + b->prim_off[bra->s2];
+    const double *e3 = b->exps + b->prim_off[ket->s1], *e4 = b->exps + b->prim_off[ket->s2];
+    const double *c1 = b->cc + b->prim_off[bra->s1], *c2 = b->cc + b->prim_off[bra->s2];
+    const double *c3 = b->cc + b->prim_off[ket->s1], *c4 = b->cc + b->prim_off[ket->s2];
+
+    double F[OR_MAXLT + 1];
+    for (int i1 = 0; i1 < n1; i1++)
+    for (int i2 = 0; i2 < n2; i2++) {
+        /* shellpair_quantities.c:28-33 */
+        const double a = e1[i1], bb = e2[i2], p = a + bb, sigP = 1.0 / p;
+        const double UP = pow(PI * sigP, 1.5) * exp(-a * bb * sigP * ab2);
+        double P[3], PA[3];
+        for (int i = 0; i < 3; i++) {
+            P[i] = (a * bra->X1[i] + bb * bra->X2[i]) * sigP;
+            PA[i] = P[i] - bra->X1[i];
+        }
+        const double ccP = c1[i1] * c2[i2];
+        for (int i3 = 0; i3 < n3; i3++)
+        for (int i4 = 0; i4 < n4; i4++) {
+            const double c = e3[i3], d = e4[i4], q = c + d, sigQ = 1.0 / q;
+            const double UQ = pow(PI * sigQ, 1.5) * exp(-c * d * sigQ * cd2);
+            double Q[3], QC[3], R[3];
+            for (int i = 0; i < 3; i++) {
+                Q[i] = (c * ket->X1[i] + d * ket->X2[i]) * sigQ;
+                QC[i] = Q[i] - ket->X1[i];
+                R[i] = P[i] - Q[i];
+            }
+            /* two_electron_fundamentals.c:40-85, scaled by (2 theta^2)^(m+1/2) */
+            const double th2 = 1.0 / (sigP + sigQ), two_th2 = 2.0 * th2;
+            const double R2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+            or_boys_one(th2 * R2, L, F);
+            const double pf = sqrt(2.0 / PI) * UP * UQ * ccP * c3[i3] * c4[i4];
+            double s = pf * sqrt(two_th2);
+            for (int m = 0; m <= L; m++) { vrr[0][0][m] = s * F[m]; s *= two_th2; }
+
+            /* Scaled VRR (two_electron_vrr.c:92-109):
+               target = Rx*kappa*zeta*b0 + R*zeta*b1 + a*zeta*(b2 - zeta*b3) + c*zeta*eta*b4
+               with the ket built first (zeta = sigma_Q/2, R = +(P-Q)) and the bra
+               second (zeta = sigma_P/2, R = -(P-Q)).  Rx*kappa*zeta = P-A (or Q-C). */
+            const double zP = 0.5 * sigP, zQ = 0.5 * sigQ;
+            for (int fl = 1; fl <= fmax; fl++) {
+                for (int F1 = cum(fl); F1 < cum(fl + 1); F1++) {
+                    const int *fc = comp_l[F1];
+                    int i = fc[0] ? 0 : (fc[1] ? 1 : 2);
+                    int fm[3] = {fc[0], fc[1], fc[2]}; fm[i]--;
+                    int Fm1 = flat(fm[0], fm[1], fm[2]);
+                    int ai = fm[i];
+                    int Fm2 = -1;
+                    if (ai > 0) { fm[i]--; Fm2 = flat(fm[0], fm[1], fm[2]); }
+                    for (int m = 0; m <= L - fl; m++) {
+                        double v = QC[i] * vrr[0][Fm1][m] + R[i] * zQ * vrr[0][Fm1][m + 1];
+                        if (Fm2 >= 0) v += ai * zQ * (vrr[0][Fm2][m] - zQ * vrr[0][Fm2][m + 1]);
+                        vrr[0][F1][m] = v;
+                    }
+                }
+            }
+            for (int el = 1; el <= emax; el++) {
+                int flo = lc - (emax - el); if (flo < 0) flo = 0;
+                for (int E1 = cum(el); E1 < cum(el + 1); E1++) {
+                    const int *ec = comp_l[E1];
+                    int i = ec[0] ? 0 : (ec[1] ? 1 : 2);
+                    int em[3] = {ec[0], ec[1], ec[2]}; em[i]--;
+                    int Em1 = flat(em[0], em[1], em[2]);
+                    int ai = em[i];
+                    int Em2 = -1;
+                    if (ai > 0) { em[i]--; Em2 = flat(em[0], em[1], em[2]); }
+                    for (int fl = flo; fl <= fmax; fl++)
+                    for (int F1 = cum(fl); F1 < cum(fl + 1); F1++) {
+                        const int *fc = comp_l[F1];
+                        int ci = fc[i];
+                        int Fm1 = -1;
+                        if (ci > 0) { int fm[3] = {fc[0], fc[1], fc[2]}; fm[i]--; Fm1 = flat(fm[0], fm[1], fm[2]); }
+                        for (int m = 0; m <= L - el - fl; m++) {
+                            double v = PA[i] * vrr[Em1][F1][m] - R[i] * zP * vrr[Em1][F1][m + 1];
+                            if (Em2 >= 0) v += ai * zP * (vrr[Em2][F1][m] - zP * vrr[Em2][F1][m + 1]);
+                            if (Fm1 >= 0) v += ci * zP * zQ * vrr[Em1][Fm1][m + 1];
+                            vrr[E1][F1][m] = v;
+                        }
+                    }
+                }
+            }
+            /* contraction of m=0 targets (two_electron_contract.c:45; cc folded in pf) */
+            for (int E1 = cum(la); E1 < cum(emax + 1); E1++)
+                for (int F1 = cum(lc); F1 < cum(fmax + 1); F1++)
+                    acc[E1][F1] += vrr[E1][F1][0];
+        }
+    }
+
+    /* HRR on contracted classes, ket first (two_electron_hrr.c:82). */
+    const int nc3 = ncart(lc), nc4 = ncart(ld), nc1 = ncart(la), nc2 = ncart(lb);
+    static __thread double hk[OR_NLEV][OR_NLEV][10];   /* [E][c flat][d flat] */
+    static __thread double tmp[OR_NLEV][10];
+    for (int E1 = cum(la); E1 < cum(emax + 1); E1++) {
+        /* level j = 0: d = s */
+        for (int C1 = cum(lc); C1 < cum(fmax + 1); C1++) hk[E1][C1][0] = acc[E1][C1];
+        for (int j = 1; j <= ld; j++) {
+            for (int cl = lc; cl <= lc + ld - j; cl++)
+            for (int C1 = cum(cl); C1 < cum(cl + 1); C1++)
+            for (int D1 = cum(j); D1 < cum(j + 1); D1++) {
+                const int *dc = comp_l[D1];
+                int i = dc[0] ? 0 : (dc[1] ? 1 : 2);
+                int dm[3] = {dc[0], dc[1], dc[2]}; dm[i]--;
+                int Dm1 = flat(dm[0], dm[1], dm[2]);
+                const int *cc_ = comp_l[C1];
+                int cp[3] = {cc_[0], cc_[1], cc_[2]}; cp[i]++;
+                int Cp1 = flat(cp[0], cp[1], cp[2]);
+                tmp[C1][D1] = hk[E1][Cp1][Dm1] + CD[i] * hk[E1][C1][Dm1];
+            }
+            for (int cl = lc; cl <= lc + ld - j; cl++)
+            for (int C1 = cum(cl); C1 < cum(cl + 1); C1++)
+            for (int D1 = cum(j); D1 < cum(j + 1); D1++) hk[E1][C1][D1] = tmp[C1][D1];
+        }
+    }
+    /* bra HRR for every final (c,d) */
+    static __thread double hb[OR_NLEV][10], tb[OR_NLEV][10];
+    for (int c3 = 0; c3 < nc3; c3++)
+    for (int c4 = 0; c4 < nc4; c4++) {
+        int C1 = cum(lc) + c3, D1 = cum(ld) + c4;
+        for (int E1 = cum(la); E1 < cum(emax + 1); E1++) hb[E1][0] = hk[E1][C1][D1];
+        for (int j = 1; j <= lb; j++) {
+            for (int al = la; al <= la + lb - j; al++)
+            for (int A1 = cum(al); A1 < cum(al + 1); A1++)
+            for (int B1 = cum(j); B1 < cum(j + 1); B1++) {
+                const int *bc = comp_l[B1];
+                int i = bc[0] ? 0 : (bc[1] ? 1 : 2);
+                int bm[3] = {bc[0], bc[1], bc[2]}; bm[i]--;
+                int Bm1 = flat(bm[0], bm[1], bm[2]);
+                const int *ac = comp_l[A1];
+                int ap[3] = {ac[0], ac[1], ac[2]}; ap[i]++;
+                int Ap1 = flat(ap[0], ap[1], ap[2]);
+                tb[A1][B1] = hb[Ap1][Bm1] + AB[i] * hb[A1][Bm1];
+            }
+            for (int al = la; al <= la + lb - j; al++)
+            for (int A1 = cum(al); A1 < cum(al + 1); A1++)
+            for (int B1 = cum(j); B1 < cum(j + 1); B1++) hb[A1][B1] = tb[A1][B1];
+        }
+        for (int a1 = 0; a1 < nc1; a1++)
+        for (int b1 = 0; b1 < nc2; b1++) {
+            int A1 = cum(la) + a1, B1 = cum(lb) + b1;
+            const int *ac = comp_l[A1], *bc = comp_l[B1], *cc_ = comp_l[C1], *dc = comp_l[D1];
+            double nrm = comp_norm(ac[0], ac[1], ac[2]) * comp_norm(bc[0], bc[1], bc[2])
+                       * comp_norm(cc_[0], cc_[1], cc_[2]) * comp_norm(dc[0], dc[1], dc[2]);
+            out[((a1 * nc2 + b1) * nc3 + c3) * nc4 + c4] = hb[A1][B1] * nrm;
+        }
+    }
+}
+
+/* Final-basis block in caller order (sa sb | sc sd). */
+static void eri_block(const basis_t *b, int sa, int sb, int sc, int sd, double *out)
+{
+    init_tables();
+    /* swap bra/ket when Ltot(ket) > Ltot(bra): Methods/integrals.py:432-435 */
+    int qswap = (b->l[sc] + b->l[sd]) > (b->l[sa] + b->l[sb]);
+    int s[4] = {sa, sb, sc, sd};
+    if (qswap) { s[0] = sc; s[1] = sd; s[2] = sa; s[3] = sb; }
+    pair_view bra, ket;
+    make_view(b, s[0], s[1], &bra);
+    make_view(b, s[2], s[3], &ket);
+    double cart[1296], full[1296];
+    eri_cart_view(b, &bra, &ket, cart);
+    /* un-goofy: reorder to (s0 s1 | s2 s3) cartesian */
+    int n[4] = {ncart(b->l[s[0]]), ncart(b->l[s[1]]), ncart(b->l[s[2]]), ncart(b->l[s[3]])};
+    int nv1 = ncart(bra.l1), nv2 = ncart(bra.l2), nv3 = ncart(ket.l1), nv4 = ncart(ket.l2);
+    for (int i0 = 0; i0 < n[0]; i0++) for (int i1 = 0; i1 < n[1]; i1++)
+    for (int i2 = 0; i2 < n[2]; i2++) for (int i3 = 0; i3 < n[3]; i3++) {
+        int v1 = bra.swapped ? i1 : i0, v2 = bra.swapped ? i0 : i1;
+        int v3 = ket.swapped ? i3 : i2, v4 = ket.swapped ? i2 : i3;
+        full[((i0 * n[1] + i1) * n[2] + i2) * n[3] + i3] = cart[((v1 * nv2 + v2) * nv3 + v3) * nv4 + v4];
+    }
+    (void)nv1;
+    /* spherical transform cs_P . X . cs_Q^T (Methods/integrals.py:547) */
+    int m[4]; double T[4][36];
+    for (int k = 0; k < 4; k++) {
+        int sh = s[k];
+        m[k] = b->spher[sh] ? 2 * b->l[sh] + 1 : n[k];
+        if (b->spher[sh]) spher_rows(b->l[sh], T[k]);
+        else { for (int i = 0; i < n[k] * n[k]; i++) T[k][i] = 0; for (int i = 0; i < n[k]; i++) T[k][i * n[k] + i] = 1; }
+    }
+    double sph[1296];
+    for (int i0 = 0; i0 < m[0]; i0++) for (int i1 = 0; i1 < m[1]; i1++)
+    for (int i2 = 0; i2 < m[2]; i2++) for (int i3 = 0; i3 < m[3]; i3++) {
+        double acc = 0;
+        for (int j0 = 0; j0 < n[0]; j0++) { double t0 = T[0][i0 * n[0] + j0]; if (t0 == 0) continue;
+        for (int j1 = 0; j1 < n[1]; j1++) { double t1 = t0 * T[1][i1 * n[1] + j1]; if (t1 == 0) continue;
+        for (int j2 = 0; j2 < n[2]; j2++) { double t2 = t1 * T[2][i2 * n[2] + j2]; if (t2 == 0) continue;
+        for (int j3 = 0; j3 < n[3]; j3++) { double t3 = t2 * T[3][i3 * n[3] + j3]; if (t3 == 0) continue;
+            acc += t3 * full[((j0 * n[1] + j1) * n[2] + j2) * n[3] + j3]; }}}}
+        sph[((i0 * m[1] + i1) * m[2] + i2) * m[3] + i3] = acc;
+    }
+    /* un-swap bra/ket (.T.reshape, Methods/integrals.py:552-553) */
+    if (!qswap) { memcpy(out, sph, sizeof(double) * m[0] * m[1] * m[2] * m[3]); return; }
+    for (int i0 = 0; i0 < m[0]; i0++) for (int i1 = 0; i1 < m[1]; i1++)
+    for (int i2 = 0; i2 < m[2]; i2++) for (int i3 = 0; i3 < m[3]; i3++)
+        out[((i2 * m[3] + i3) * m[0] + i0) * m[1] + i1] = sph[((i0 * m[1] + i1) * m[2] + i2) * m[3] + i3];
+}
+
+#define BASIS_ARGS int nshell, const int *l, const int *nprim, const int *prim_off, const int *spher, \
+                   const double *center, const double *exps, const double *cc
+#define BASIS_INIT(bv) basis_t bv; int ao_off_[4096], nfun_[4096]; \
+    bv.nshell = nshell; bv.l = l; bv.nprim = nprim; bv.prim_off = prim_off; bv.spher = spher; \
+    bv.center = center; bv.exps = exps; bv.cc = cc; bv.ao_off = ao_off_; bv.nfun = nfun_; basis_setup(&bv)
+
+int or_nao(BASIS_ARGS)
+{
+    if (nshell > 4096) return -1;
+    BASIS_INIT(b);
+    return b.nao;
+}
+
+/* One final-basis ERI block for (sa sb|sc sd); Methods/integrals.py:427-555. */
+int or_eri_quartet(BASIS_ARGS, int sa, int sb, int sc, int sd, double *out)
+{
+    if (nshell > 4096) return -1;
+    for (int s = 0; s < nshell; s++) if (l[s] > OR_MAXL || nprim[s] > OR_MAXK) return -2;
+    BASIS_INIT(b);
+    eri_block(&b, sa, sb, sc, sd, out);
+    return 0;
+}
+
+/* Schwarz bounds per shell pair, Methods/hartree_fock.py:244-254: the bound
+   of pair (a,b) is the max over its functions of sqrt((mn|mn)). */
+int or_schwarz(BASIS_ARGS, double *pair_bound /* nshell*nshell */)
+{
+    if (nshell > 4096) return -1;
+    BASIS_INIT(b);
+    init_tables();
+    #pragma omp parallel for schedule(dynamic)
+    for (int a = 0; a < nshell; a++) {
+        double blk[1296];
+        for (int c = a; c < nshell; c++) {
+            eri_block(&b, a, c, a, c, blk);
+            int na = b.nfun[a], nc = b.nfun[c];
+            double mx = 0;
+            for (int m = 0; m < na; m++) for (int n = 0; n < nc; n++) {
+                double v = blk[((m * nc + n) * na + m) * nc + n];
+                double s = sqrt(v > 0 ? v : 0);
+                if (s > mx) mx = s;
+            }
+            pair_bound[a * nshell + c] = pair_bound[c * nshell + a] = mx;
+        }
+    }
+    return 0;
+}
+
+/* Dense screened ERI tensor out[N][N][N][N] (must be zero on entry), with the
+   reference's rule: diagonal blocks (ab|ab) always kept, other quartets kept
+   when max(bound_ab) * max(bound_cd) > thresh (Methods/hartree_fock.py:290-297),
+   scattered with 8-fold symmetry (:314-325).  Returns the number of shell
+   quartets computed. */
+long long or_eri_dense(BASIS_ARGS, double thresh, const double *pair_bound, double *out)
+{
+    if (nshell > 4096) return -1;
+    BASIS_INIT(b);
+    init_tables();
+    const size_t N = b.nao;
+    long long npair = (long long)nshell * (nshell + 1) / 2;
+    int *pa = malloc(sizeof(int) * npair), *pb = malloc(sizeof(int) * npair);
+    long long k = 0;
+    for (int a = 0; a < nshell; a++) for (int c = a; c < nshell; c++) { pa[k] = a; pb[k] = c; k++; }
+    long long nq = 0;
+    #pragma omp parallel for schedule(dynamic) reduction(+:nq)
+    for (long long P = 0; P < npair; P++) {
+        double blk[1296];
+        int a = pa[P], bb = pb[P];
+        double qab = pair_bound[a * nshell + bb];
+        for (long long Qi = P; Qi < npair; Qi++) {
+            int c = pa[Qi], d = pb[Qi];
+            if (Qi != P && !(qab * pair_bound[c * nshell + d] > thresh)) continue;
+            eri_block(&b, a, bb, c, d, blk);
+            nq++;
+            int na = b.nfun[a], nb = b.nfun[bb], nc = b.nfun[c], nd = b.nfun[d];
+            int oa = b.ao_off[a], ob = b.ao_off[bb], oc = b.ao_off[c], od = b.ao_off[d];
+            for (int i = 0; i < na; i++) for (int j = 0; j < nb; j++)
+            for (int kk = 0; kk < nc; kk++) for (int ll = 0; ll < nd; ll++) {
+                double v = blk[((i * nb + j) * nc + kk) * nd + ll];
+                size_t m = oa + i, n = ob + j, lm = oc + kk, s = od + ll;
+                out[((m * N + n) * N + lm) * N + s] = v;
+                out[((n * N + m) * N + lm) * N + s] = v;
+                out[((m * N + n) * N + s) * N + lm] = v;
+                out[((n * N + m) * N + s) * N + lm] = v;
+                out[((lm * N + s) * N + m) * N + n] = v;
+                out[((lm * N + s) * N + n) * N + m] = v;
+                out[((s * N + lm) * N + m) * N + n] = v;
+                out[((s * N + lm) * N + n) * N + m] = v;
+            }
+        }
+    }
+    free(pa); free(pb);
+    return nq;
+}
+
+/* J_ab = sum_cd Dt_cd (ab|cd);  K_ad = -sum_cb D_cb (ab|cd)
+   (Methods/hartree_fock.py:345-347; K carries the reference's minus sign). */
+void or_jk(int n, const double *eri, const double *Dt, const double *D, double *J, double *K)
+{
+    const size_t N = n;
+    #pragma omp parallel for schedule(static)
+    for (int a = 0; a < n; a++) {
+        for (size_t x = 0; x < N; x++) { J[a * N + x] = 0; K[a * N + x] = 0; }
+        for (size_t bq = 0; bq < N; bq++) {
+            const double *blk = eri + ((size_t)a * N + bq) * N * N;
+            double j = 0;
+            for (size_t c = 0; c < N; c++) {
+                const double *row = blk + c * N;
+                double dcb = D[c * N + bq];
+                for (size_t d = 0; d < N; d++) {
+                    j += Dt[c * N + d] * row[d];
+                    K[a * N + d] -= dcb * row[d];
+                }
+            }
+            J[a * N + bq] = j;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* One-electron S, T+V (final basis), Methods/integrals.py:220-370.          */
+/* ------------------------------------------------------------------------ */
+/* overlap-like arrays over levels: primitive S[E][Bflat] for HRR */
+static void onee_block(const basis_t *b, int sa, int sb, int natom, const double *axyz, const double *Z,
+                       double *Sout, double *Tout, double *Vout /* cart, caller order */)
+{
+    static const double PI = 3.14159265358979323846;
+    pair_view v;
+    make_view(b, sa, sb, &v);
+    const int la = v.l1, lb = v.l2;
+    const int n1 = b->nprim[v.s1], n2 = b->nprim[v.s2];
+    const double *e1 = b->exps + b->prim_off[v.s1], *e2 = b->exps + b->prim_off[v.s2];
+    const double *c1 = b->cc + b->prim_off[v.s1], *c2 = b->cc + b->prim_off[v.s2];
+    double AB[3];
+    for (int i = 0; i < 3; i++) AB[i] = v.X1[i] - v.X2[i];
+    const double ab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+    const int nc1 = ncart(la), nc2 = ncart(lb);
+    double Sc[36] = {0}, Tc[36] = {0}, Vc[36] = {0};
+    /* overlap needed up to (la | lb+2): VRR levels la..la+lb+2 */
+    const int emaxS = la + lb + 2, emaxV = la + lb;
+    double s_v[84];                 /* levels 0..6 */
+    double h[84][35];               /* HRR [a flat][b flat] up to b level lb+2 (<=4) */
+    double tmp[84][35];
+    double vv[OR_NLEV][OR_MAXLT + 1];
+    double F[OR_MAXLT + 1];
+    for (int i1 = 0; i1 < n1; i1++)
+    for (int i2 = 0; i2 < n2; i2++) {
+        const double a = e1[i1], bet = e2[i2], p = a + bet, sig = 1.0 / p;
+        const double U = pow(PI * sig, 1.5) * exp(-a * bet * sig * ab2);
+        const double ccp = c1[i1] * c2[i2];
+        double P[3], PA[3];
+        for (int i = 0; i < 3; i++) { P[i] = (a * v.X1[i] + bet * v.X2[i]) * sig; PA[i] = P[i] - v.X1[i]; }
+        /* overlap VRR: one_electron_vrr.c:59-65 */
+        s_v[0] = U;
+        for (int el = 1; el <= emaxS; el++)
+            for (int E1 = cum(el); E1 < cum(el + 1); E1++) {
+                const int *ec = comp_l[E1];
+                int i = ec[0] ? 0 : (ec[1] ? 1 : 2);
+                int em[3] = {ec[0], ec[1], ec[2]}; em[i]--;
+                int ai = em[i];
+                double val = PA[i] * s_v[flat(em[0], em[1], em[2])];
+                if (ai > 0) { em[i]--; val += ai * 0.5 * sig * s_v[flat(em[0], em[1], em[2])]; }
+                s_v[E1] = val;
+            }
+        /* overlap HRR up to b level lb+2: one_electron_hrr.c:64 */
+        for (int A1 = cum(la); A1 < cum(emaxS + 1); A1++) h[A1][0] = s_v[A1];
+        for (int j = 1; j <= lb + 2; j++) {
+            for (int al = la; al <= emaxS - j; al++)
+            for (int A1 = cum(al); A1 < cum(al + 1); A1++)
+            for (int B1 = cum(j); B1 < cum(j + 1); B1++) {
+                const int *bc = comp_l[B1];
+                int i = bc[0] ? 0 : (bc[1] ? 1 : 2);
+                int bm[3] = {bc[0], bc[1], bc[2]}; bm[i]--;
+                const int *ac = comp_l[A1];
+                int ap[3] = {ac[0], ac[1], ac[2]}; ap[i]++;
+                tmp[A1][B1] = h[flat(ap[0], ap[1], ap[2])][flat(bm[0], bm[1], bm[2])] + AB[i] * h[A1][flat(bm[0], bm[1], bm[2])];
+            }
+            for (int al = la; al <= emaxS - j; al++)
+            for (int A1 = cum(al); A1 < cum(al + 1); A1++)
+            for (int B1 = cum(j); B1 < cum(j + 1); B1++) h[A1][B1] = tmp[A1][B1];
+        }
+        /* overlap + kinetic (one_electron_kinetic.c:60-64) at primitive level */
+        for (int x1 = 0; x1 < nc1; x1++)
+        for (int x2 = 0; x2 < nc2; x2++) {
+            int A1 = cum(la) + x1, B1 = cum(lb) + x2;
+            const int *bc = comp_l[B1];
+            double sab = h[A1][B1];
+            double t = bet * (2 * lb + 3) * sab;
+            for (int i = 0; i < 3; i++) {
+                int bp[3] = {bc[0], bc[1], bc[2]}; bp[i] += 2;
+                t -= 2 * bet * bet * h[A1][flat(bp[0], bp[1], bp[2])];
+                if (bc[i] > 1) {
+                    int bm[3] = {bc[0], bc[1], bc[2]}; bm[i] -= 2;
+                    t -= 0.5 * bc[i] * (bc[i] - 1) * h[A1][flat(bm[0], bm[1], bm[2])];
+                }
+            }
+            Sc[x1 * nc2 + x2] += ccp * sab;
+            Tc[x1 * nc2 + x2] += ccp * t;
+        }
+        /* nuclear attraction per atom: one_electron_fundamentals.c:31-92, vrr :66-73 */
+        for (int at = 0; at < natom; at++) {
+            double PC[3], R2 = 0;
+            for (int i = 0; i < 3; i++) { PC[i] = P[i] - axyz[3 * at + i]; R2 += PC[i] * PC[i]; }
+            or_boys_one(p * R2, emaxV, F);
+            const double spf = -Z[at] * sqrt(2.0 / PI) * U * sqrt(2.0 * p);
+            for (int m = 0; m <= emaxV; m++) vv[0][m] = spf * F[m];
+            for (int el = 1; el <= emaxV; el++)
+                for (int E1 = cum(el); E1 < cum(el + 1); E1++) {
+                    const int *ec = comp_l[E1];
+                    int i = ec[0] ? 0 : (ec[1] ? 1 : 2);
+                    int em[3] = {ec[0], ec[1], ec[2]}; em[i]--;
+                    int Em1 = flat(em[0], em[1], em[2]);
+                    int ai = em[i], Em2 = -1;
+                    if (ai > 0) { em[i]--; Em2 = flat(em[0], em[1], em[2]); }
+                    for (int m = 0; m <= emaxV - el; m++) {
+                        double val = PA[i] * vv[Em1][m] - PC[i] * vv[Em1][m + 1];
+                        if (Em2 >= 0) val += ai * 0.5 * sig * (vv[Em2][m] - vv[Em2][m + 1]);
+                        vv[E1][m] = val;
+                    }
+                }
+            double hv[OR_NLEV][10], tv[OR_NLEV][10];
+            for (int A1 = cum(la); A1 < cum(emaxV + 1); A1++) hv[A1][0] = vv[A1][0];
+            for (int j = 1; j <= lb; j++) {
+                for (int al = la; al <= emaxV - j; al++)
+                for (int A1 = cum(al); A1 < cum(al + 1); A1++)
+                for (int B1 = cum(j); B1 < cum(j + 1); B1++) {
+                    const int *bc = comp_l[B1];
+                    int i = bc[0] ? 0 : (bc[1] ? 1 : 2);
+                    int bm[3] = {bc[0], bc[1], bc[2]}; bm[i]--;
+                    const int *ac = comp_l[A1];
+                    int ap[3] = {ac[0], ac[1], ac[2]}; ap[i]++;
+                    tv[A1][B1] = hv[flat(ap[0], ap[1], ap[2])][flat(bm[0], bm[1], bm[2])] + AB[i] * hv[A1][flat(bm[0], bm[1], bm[2])];
+                }
+                for (int al = la; al <= emaxV - j; al++)
+                for (int A1 = cum(al); A1 < cum(al + 1); A1++)
+                for (int B1 = cum(j); B1 < cum(j + 1); B1++) hv[A1][B1] = tv[A1][B1];
+            }
+            for (int x1 = 0; x1 < nc1; x1++)
+            for (int x2 = 0; x2 < nc2; x2++)
+                Vc[x1 * nc2 + x2] += ccp * hv[cum(la) + x1][cum(lb) + x2];
+        }
+    }
+    /* normalisation (Methods/integrals.py:353-356) and un-goofy transpose */
+    int na = ncart(b->l[sa]), nb = ncart(b->l[sb]);
+    for (int i = 0; i < na; i++) for (int j = 0; j < nb; j++) {
+        int x1 = v.swapped ? j : i, x2 = v.swapped ? i : j;
+        const int *c1_ = comp_l[cum(la) + x1], *c2_ = comp_l[cum(lb) + x2];
+        double nrm = comp_norm(c1_[0], c1_[1], c1_[2]) * comp_norm(c2_[0], c2_[1], c2_[2]);
+        Sout[i * nb + j] = Sc[x1 * nc2 + x2] * nrm;
+        Tout[i * nb + j] = Tc[x1 * nc2 + x2] * nrm;
+        Vout[i * nb + j] = Vc[x1 * nc2 + x2] * nrm;
+    }
+}
+
+/* S, T, V full matrices in the final basis (N x N each). */
+int or_one_electron(BASIS_ARGS, int natom, const double *axyz, const double *Z,
+                    double *S, double *T, double *V)
+{
+    if (nshell > 4096) return -1;
+    BASIS_INIT(b);
+    init_tables();
+    const int N = b.nao;
+    #pragma omp parallel for schedule(dynamic)
+    for (int a = 0; a < nshell; a++) {
+        double s[36], t[36], v[36], ts[36];
+        for (int c = a; c < nshell; c++) {
+            onee_block(&b, a, c, natom, axyz, Z, s, t, v);
+            int na = ncart(l[a]), nc = ncart(l[c]);
+            int ma = b.nfun[a], mc = b.nfun[c];
+            double Ta[36], Tc[36];
+            if (spher[a]) spher_rows(l[a], Ta); else { for (int i = 0; i < na * na; i++) Ta[i] = 0; for (int i = 0; i < na; i++) Ta[i * na + i] = 1; }
+            if (spher[c]) spher_rows(l[c], Tc); else { for (int i = 0; i < nc * nc; i++) Tc[i] = 0; for (int i = 0; i < nc; i++) Tc[i * nc + i] = 1; }
+            double *mats[3] = {s, t, v};
+            double *outs[3] = {S, T, V};
+            for (int k = 0; k < 3; k++) {
+                /* cs_A . X . cs_B^T (Methods/integrals.py:362-363) */
+                for (int i = 0; i < ma; i++) for (int j = 0; j < mc; j++) {
+                    double acc = 0;
+                    for (int x = 0; x < na; x++) for (int y = 0; y < nc; y++)
+                        acc += Ta[i * na + x] * mats[k][x * nc + y] * Tc[j * nc + y];
+                    ts[i * mc + j] = acc;
+                }
+                for (int i = 0; i < ma; i++) for (int j = 0; j < mc; j++) {
+                    outs[k][(size_t)(b.ao_off[a] + i) * N + b.ao_off[c] + j] = ts[i * mc + j];
+                    outs[k][(size_t)(b.ao_off[c] + j) * N + b.ao_off[a] + i] = ts[i * mc + j];
+                }
+            }
+        }
+    }
+    return 0;
+}

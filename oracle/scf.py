@@ -1,0 +1,211 @@
+"""numpy restatement of the reference SCF / DIIS / MP2 drivers (TEST
+INFRASTRUCTURE ONLY -- imported by tests/, smoke() and bench's cpu_baseline).
+
+Follows:
+* hartree_fock.do                 Methods/hartree_fock.py:23-154
+* make_core_matrices (canonical X) Methods/hartree_fock.py:207-237
+* make_coulomb_exchange_matrices  Methods/hartree_fock.py:329-347 (einsums)
+* make_fock_matrices              Methods/hartree_fock.py:352-355
+* make_MOs / densities / energy   Methods/hartree_fock.py:168-201
+* DIIS (C1)                       Methods/hf_extensions/diis.py:14-155
+* sort_occupied                   Util/structures.py:895-901
+* MP2                             Methods/mp2.py:16-120 (spin-orbital sums);
+  plus the closed-shell O(N^5) form used as an independent identity check.
+"""
+import numpy as np
+
+from . import cint
+
+LINEAR_DEPENDENCE = 1e-6
+
+
+def orthogonalizer(S):
+    """Canonical X = U s^-1/2 over s > 1e-6, eigenvalues descending."""
+    s, U = np.linalg.eigh(S)
+    order = s.argsort()[::-1]
+    U = U[:, order]
+    s = s[order]
+    keep = s > LINEAR_DEPENDENCE
+    return U[:, keep] * (s[keep] ** -0.5)
+
+
+class _DIIS:
+    def __init__(self):
+        self.focks, self.res, self.matrix = [], [], None
+
+
+def _residual(S, D, F, X):
+    r = S @ D @ F - F @ D @ S
+    t = X.T @ r @ X
+    out = np.zeros_like(r)
+    n = t.shape[0]
+    out[:n, :n] = t
+    return out
+
+
+def _diis_step(res, F, st, threshold=0.5, max_cond=1e7):
+    """Methods/hf_extensions/diis.py:32-41 with C1 coefficients."""
+    if abs(res.max()) < abs(threshold):
+        st.res.append(res)
+        st.focks.append(F)
+        if len(st.res) > 1:
+            size = len(st.res) + 1
+            new = np.full((size, size), -1.0)
+            new[-1, -1] = 0.0
+            if st.matrix is None:
+                for i, r1 in enumerate(st.res):
+                    for j, r2 in enumerate(st.res):
+                        new[i, j] = np.trace(r1 @ r2.T)
+            else:
+                new[:size - 1, :size - 1] = st.matrix
+            vec = [np.trace(st.res[-1] @ r.T) for r in st.res]
+            row = size - 2
+            new[:len(vec), row] = vec
+            new[row, :len(vec)] = vec
+            st.matrix = new
+            # reduce_space: the size clause is always False in the reference
+            def cond(m):
+                ev = np.abs(np.linalg.eigvals(m))
+                return np.inf if ev.min() == 0 else ev.max() / ev.min()
+            c = cond(st.matrix)
+            while c > max_cond:
+                if len(st.res) == 1:
+                    break
+                st.matrix = np.delete(np.delete(st.matrix, 0, 0), 0, 1)
+                st.res.pop(0)
+                st.focks.pop(0)
+                c = cond(st.matrix)
+            rhs = np.zeros(len(st.matrix))
+            rhs[-1] = -1.0
+            coeffs = np.linalg.solve(st.matrix, rhs)[:-1]
+            F = sum(c * f for c, f in zip(coeffs, st.focks))
+    return F
+
+
+def scf(H, S, eri, nalpha, nbeta, enuc, conv=1e-7, maxiter=50, use_diis=True, guess=None,
+        jk=None):
+    """UHF/RHF SCF exactly as the reference loop.  ``jk(Dt, Da, Db)`` may be
+    supplied to replace the dense einsums; it returns (J, Ka, Kb) with the
+    reference's sign convention (K negative)."""
+    X = orthogonalizer(S)
+
+    def make_mos(F):
+        e, C = np.linalg.eigh(X.T @ F @ X)
+        return e, X @ C
+
+    def sort_occ(e, C, n):
+        idx = np.argsort(e[:n])
+        idx = np.append(idx, np.arange(n, len(e)))
+        return e[idx], C[:, idx]
+
+    if jk is None:
+        def jk(Dt, Da, Db):
+            J, Ka = cint.jk(eri, Dt, Da)
+            if Db is Da:
+                return J, Ka, Ka
+            _, Kb = cint.jk(eri, Dt, Db)
+            return J, Ka, Kb
+
+    if guess is None:
+        ea, Ca = make_mos(H)
+        eb, Cb = ea, Ca
+    else:
+        Ca = Cb = guess
+        ea = eb = np.zeros(H.shape[0])
+    Da = Ca[:, :nalpha] @ Ca[:, :nalpha].T
+    Db = Cb[:, :nbeta] @ Cb[:, :nbeta].T
+    Fa = Fb = H
+    E = 0.5 * (np.sum((Da + Db) * H) + np.sum(Da * Fa) + np.sum(Db * Fb))
+    dE = 1.0
+    it = 0
+    da, db = _DIIS(), _DIIS()
+    history = []
+    converged = False
+    while conv < abs(dE):
+        it += 1
+        if it > maxiter:
+            break
+        Dt = Da + Db
+        J, Ka, Kb = jk(Dt, Da, Db if nalpha != nbeta else Da)
+        Fa = H + J + Ka
+        Fb = H + J + Kb
+        if use_diis and it > 1:
+            ra = _residual(S, Da, Fa, X)
+            rb = _residual(S, Db, Fb, X)
+            Fa = _diis_step(ra, Fa, da)
+            Fb = _diis_step(rb, Fb, db)
+        ea, Ca = make_mos(Fa)
+        eb, Cb = make_mos(Fb)
+        ea, Ca = sort_occ(ea, Ca, nalpha)
+        eb, Cb = sort_occ(eb, Cb, nbeta)
+        Da = Ca[:, :nalpha] @ Ca[:, :nalpha].T
+        Db = Cb[:, :nbeta] @ Cb[:, :nbeta].T
+        old = E
+        E = 0.5 * (np.sum((Da + Db) * H) + np.sum(Da * Fa) + np.sum(Db * Fb))
+        dE = E - old
+        history.append(E + enuc)
+    converged = abs(dE) <= conv
+    return dict(energy=E + enuc, electronic=E, iterations=it, converged=converged,
+                Ca=Ca, Cb=Cb, ea=ea, eb=eb, Da=Da, Db=Db, history=history, X=X)
+
+
+def nuclear_repulsion(mol):
+    """Methods/integrals.py:196-207."""
+    e = 0.0
+    for i, a in enumerate(mol.Atoms):
+        for j in range(i):
+            b = mol.Atoms[j]
+            r = np.linalg.norm(np.array(a.Coordinates) - np.array(b.Coordinates))
+            e += a.NuclearCharge * b.NuclearCharge / r
+    return e
+
+
+def run_rhf(mol, conv=1e-10, maxiter=100, thresh=1e-8):
+    S, T, V = cint.one_electron(mol)
+    H = T + V
+    eri, nq = cint.eri_dense(mol, thresh)
+    res = scf(H, S, eri, mol.NAlphaElectrons, mol.NBetaElectrons, nuclear_repulsion(mol),
+              conv=conv, maxiter=maxiter)
+    res.update(H=H, S=S, eri=eri, nquartets=nq)
+    return res
+
+
+def mp2_closed_shell(eri, C, e, nocc):
+    """E = sum (ia|jb)[2(ia|jb) - (ib|ja)] / (e_i + e_j - e_a - e_b), with the
+    O(N^5) quarter transforms."""
+    Co, Cv = C[:, :nocc], C[:, nocc:]
+    t = np.einsum("mnls,mi->inls", eri, Co, optimize=True)
+    t = np.einsum("inls,na->ials", t, Cv, optimize=True)
+    t = np.einsum("ials,lj->iajs", t, Co, optimize=True)
+    ovov = np.einsum("iajs,sb->iajb", t, Cv, optimize=True)
+    eo, ev = e[:nocc], e[nocc:]
+    den = eo[:, None, None, None] - ev[None, :, None, None] + eo[None, None, :, None] - ev[None, None, None, :]
+    return float(np.sum(ovov * (2.0 * ovov - ovov.transpose(0, 3, 2, 1)) / den)), ovov
+
+
+def mp2_spin_orbital(eri, Ca, Cb, ea, eb, na, nb):
+    """Methods/mp2.py:37-109 (full transform, then the same-spin sums over
+    i>=j, p>=q and opposite-spin sums over all i,j,p,q)."""
+    def tr(C1, C2):
+        return np.einsum("mnls,ma,nb,lc,sd->abcd", eri, C1, C1, C2, C2, optimize=True)
+    A = tr(Ca, Ca)
+    B = tr(Cb, Cb)
+    AB = tr(Ca, Cb)
+    N = eri.shape[0]
+    eaa = ebb = eab = 0.0
+    for i in range(na):
+        for j in range(i + 1):
+            for p in range(na, N):
+                for q in range(na, p + 1):
+                    eaa += (A[i, p, j, q] - A[i, q, j, p]) ** 2 / (ea[i] + ea[j] - ea[p] - ea[q])
+    for i in range(nb):
+        for j in range(i + 1):
+            for p in range(nb, N):
+                for q in range(nb, p + 1):
+                    ebb += (B[i, p, j, q] - B[i, q, j, p]) ** 2 / (eb[i] + eb[j] - eb[p] - eb[q])
+    for i in range(na):
+        for j in range(nb):
+            for p in range(na, N):
+                for q in range(nb, N):
+                    eab += AB[i, p, j, q] ** 2 / (ea[i] + eb[j] - ea[p] - eb[q])
+    return eaa + eab + ebb
