@@ -1,0 +1,101 @@
+/* pychem_hip.h -- C ABI of the MI355X integral + Fock-build engine.
+ *
+ * Drop-in replacement for the reference's `_c_ints` CPython extension
+ * (Methods/_c_ints.c:68-81) and the per-class driver loops above it
+ * (Methods/integrals.py, Methods/hartree_fock.py).  Where the reference
+ * crosses Python->C once per recurrence class of one shell quartet, these
+ * entry points are batched and device resident: the basis is uploaded once,
+ * integrals stay in HBM, and the SCF loop calls pq_jk once per iteration.
+ *
+ * Conventions: every function returns 0 on success and a negative value on
+ * failure (message via pq_last_error()).  Pointers named *_dev / matrices
+ * are device pointers; `stream` is a hipStream_t (NULL = default stream).
+ * Matrices are row-major float64 over Cartesian basis functions in the
+ * reference's global order (atoms, shells in basis order, components lx desc
+ * then ly desc).  Coordinates in bohr.  ERIs in chemists' notation (ab|cd).
+ */
+#ifndef PYCHEM_HIP_H
+#define PYCHEM_HIP_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pq_basis pq_basis;
+
+/* Last error message of the calling thread. */
+const char* pq_last_error(void);
+/* ABI version (1). */
+int pq_version(void);
+/* Number of visible GPUs. */
+int pq_device_count(void);
+
+/* Boys function F_m(T), m = 0..mmax (mmax <= 8), out[n][mmax+1].
+ * Replaces the interpolation in Methods/c_ints/two_electron_fundamentals.c:60-85
+ * and one_electron_fundamentals.c:57-96. */
+int pq_boys(int n, const double* T_dev, int mmax, double* out_dev, void* stream);
+
+/* Upload shells (l <= 2) and atoms; builds all shell pairs a <= b with their
+ * primitive-pair quantities.  Replaces the per-pair
+ * _c_ints.shellpair_quantities call (Methods/_c_ints.c:114-155, called from
+ * Util/structures.py:931).  cc = contraction coefficients already scaled by
+ * (2 alpha)^((l+1.5)/2) (Util/structures.py:843). */
+int pq_basis_create(int nshell, const int* l, const int* nprim, const int* prim_off,
+                    const double* center, const double* exps, const double* cc,
+                    int natom, const double* atom_xyz, const double* atom_z,
+                    pq_basis** out);
+void pq_basis_destroy(pq_basis* b);
+/* Mark spherical (5d) shells, spher[nshell] in {0,1} (Util/structures.py:844-849).
+ * Only affects Schwarz bounds, which the reference takes in the final basis;
+ * all matrices stay Cartesian and the caller applies cart->spherical. */
+int pq_basis_set_spherical(pq_basis* b, const int* spher);
+/* info[0..3] = ncart, nshell, npair, nprimpair. */
+int pq_basis_info(const pq_basis* b, long long* info);
+
+/* Overlap, kinetic and nuclear-attraction matrices (Cartesian, normalised).
+ * Replaces integrals.one_electron over all pairs (Methods/integrals.py:220-370)
+ * i.e. _c_ints.one_electron_{fundamentals,vrr,hrr,kinetic,contract}
+ * (Methods/_c_ints.c:380-644), as used by make_core_matrices
+ * (Methods/hartree_fock.py:207-223). */
+int pq_one_electron(pq_basis* b, double* S_dev, double* T_dev, double* V_dev, void* stream);
+
+/* Schwarz bounds per shell pair: max over functions of sqrt((mn|mn))
+ * (Methods/hartree_fock.py:244-254).  pair_bound_host (nshell*nshell) may be
+ * NULL. */
+int pq_schwarz(pq_basis* b, double* pair_bound_host, void* stream);
+
+/* Screened unique ERIs into HBM: quartets with bound product > threshold
+ * (plus every diagonal (P|P)), Methods/hartree_fock.py:241-325 with the
+ * per-quartet integrals.two_electron (Methods/integrals.py:427-555, i.e.
+ * _c_ints.two_electron_{fundamentals,vrr,contract,hrr}, Methods/_c_ints.c:
+ * 157-378).  Quartets whose bound product is below fp32_threshold are
+ * computed and stored in FP32 (0 disables the split). */
+int pq_eri_build(pq_basis* b, double threshold, double fp32_threshold, void* stream);
+/* Recompute the stored ERIs over the same quartet lists. */
+int pq_eri_recompute(pq_basis* b, void* stream);
+/* Number of ERI classes (21 for s..d). */
+int pq_eri_nclass(void);
+/* Per class c: stats[6c..6c+5] = nq_fp64, nq_fp32, components per quartet,
+ * primitive quartets fp64, primitive quartets fp32, class code la*1000+lb*100+lc*10+ld. */
+int pq_eri_stats(const pq_basis* b, long long* stats, int nstats);
+
+/* J and K from the stored ERIs: J_ab = sum_cd Dj_cd (ab|cd),
+ * K_ad = -sum_cb Dk_cb (ab|cd); Dj, Dk symmetric.  Replaces
+ * make_coulomb_exchange_matrices (Methods/hartree_fock.py:329-347). */
+int pq_jk(pq_basis* b, const double* Dj_dev, const double* Dk_dev, double* J_dev, double* K_dev,
+          void* stream);
+
+/* Dense (ab|cd) tensor [ncart]^4 from the stored ERIs (screened entries
+ * zero), the CoulombIntegrals array of Util/structures.py:506 as filled by
+ * Methods/hartree_fock.py:266-325. */
+int pq_eri_dense(pq_basis* b, double* out_dev, void* stream);
+
+/* Arbitrary shell quartets (shells[4*i .. 4*i+3]) -> Cartesian normalised
+ * blocks packed consecutively in out_host, each [nA][nB][nC][nD] in caller
+ * order.  The batched form of integrals.two_electron
+ * (Methods/integrals.py:427-555) before its spherical transform. */
+int pq_eri_blocks(pq_basis* b, int n, const int* shells, double* out_host, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

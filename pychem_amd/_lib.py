@@ -1,0 +1,70 @@
+"""ctypes binding of libpychem_hip.so (the C ABI in include/pychem_hip.h).
+
+The library is mandatory: if it is missing or cannot be loaded this module
+raises, there is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpychem_hip.so")
+_LIB = None
+
+c_int, c_double, c_void_p, c_longlong = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_longlong
+P_int = ctypes.POINTER(ctypes.c_int)
+P_double = ctypes.POINTER(ctypes.c_double)
+P_ll = ctypes.POINTER(ctypes.c_longlong)
+
+# name -> (restype, argtypes); mirrors include/pychem_hip.h
+SIGNATURES = {
+    "pq_last_error": (ctypes.c_char_p, []),
+    "pq_version": (c_int, []),
+    "pq_device_count": (c_int, []),
+    "pq_boys": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p]),
+    "pq_basis_create": (c_int, [c_int, P_int, P_int, P_int, P_double, P_double, P_double,
+                                c_int, P_double, P_double, ctypes.POINTER(c_void_p)]),
+    "pq_basis_destroy": (None, [c_void_p]),
+    "pq_basis_set_spherical": (c_int, [c_void_p, P_int]),
+    "pq_basis_info": (c_int, [c_void_p, P_ll]),
+    "pq_one_electron": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pq_schwarz": (c_int, [c_void_p, P_double, c_void_p]),
+    "pq_eri_build": (c_int, [c_void_p, c_double, c_double, c_void_p]),
+    "pq_eri_recompute": (c_int, [c_void_p, c_void_p]),
+    "pq_eri_nclass": (c_int, []),
+    "pq_eri_stats": (c_int, [c_void_p, P_ll, c_int]),
+    "pq_jk": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pq_eri_dense": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "pq_eri_blocks": (c_int, [c_void_p, c_int, P_int, P_double, c_void_p]),
+}
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise HipError("libpychem_hip.so not built (run python -m pychem_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = lib().pq_last_error()
+        raise HipError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+def iptr(a):
+    return a.ctypes.data_as(P_int)
+
+
+def dptr(a):
+    return a.ctypes.data_as(P_double)

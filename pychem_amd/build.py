@@ -1,0 +1,89 @@
+"""Build the HIP engine (libpychem_hip.so) for gfx950.
+
+Steps: generate the Boys table (only if missing; mpmath) and the integral
+class sources (always; gen_eri.py), compile every translation unit with hipcc
+in parallel, link one shared library next to this file.
+
+    python -m pychem_amd.build            # incremental
+    python -m pychem_amd.build --force    # rebuild everything
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+GEN = os.path.join(CSRC, "gen")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libpychem_hip.so")
+ROOT = os.path.dirname(HERE)
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-munsafe-fp-atomics", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
+         "-Wno-unused-value", "-Wno-unused-result"]
+
+
+def sources():
+    srcs = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith(".hip")]
+    srcs += [os.path.join(GEN, f) for f in sorted(os.listdir(GEN)) if f.endswith(".hip")]
+    return srcs
+
+
+def _deps(path, seen=None):
+    """Local #include "..." files of ``path``, recursively."""
+    seen = set() if seen is None else seen
+    base = os.path.dirname(path)
+    with open(path) as fh:
+        for line in fh:
+            line = line.strip()
+            if line.startswith("#include \""):
+                inc = os.path.normpath(os.path.join(base, line.split('"')[1]))
+                if os.path.exists(inc) and inc not in seen:
+                    seen.add(inc)
+                    _deps(inc, seen)
+    return seen
+
+
+def generate():
+    if not os.path.exists(os.path.join(CSRC, "boys_table.inc")):
+        subprocess.check_call([sys.executable, os.path.join(CSRC, "gen_boys_table.py")])
+    subprocess.check_call([sys.executable, os.path.join(CSRC, "gen_eri.py")], stdout=subprocess.DEVNULL)
+
+
+def _compile(src, force):
+    obj = os.path.join(OBJ, os.path.basename(src).replace(".hip", ".o"))
+    if not force and os.path.exists(obj):
+        mt = os.path.getmtime(obj)
+        if all(mt > os.path.getmtime(d) for d in [src] + sorted(_deps(src))):
+            return obj, None
+    cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        return obj, r.stderr[-4000:]
+    return obj, None
+
+
+def build(force=False, jobs=None):
+    os.makedirs(OBJ, exist_ok=True)
+    generate()
+    srcs = sources()
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    objs, errors = [], []
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for obj, err in ex.map(lambda s: _compile(s, force), srcs):
+            objs.append(obj)
+            if err:
+                errors.append(err)
+    if errors:
+        raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
+    tmp = LIB + ".tmp"
+    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(force="--force" in sys.argv))

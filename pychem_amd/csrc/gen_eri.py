@@ -1,0 +1,396 @@
+"""Generate straight-line HIP code for the integral classes (eri_classes.inc).
+
+For every shell-quartet class (la lb | lc ld) with s..d shells -- each pair in
+"primary first" orientation (the VRR runs on the shell of larger l, as the
+reference's goofy handling does, Methods/integrals.py:79-85) and the bra
+pair class >= the ket pair class -- it emits one kernel:
+
+  * loop over bra and ket primitive pairs (runtime trip counts),
+  * Boys F_0..F_L, then the Obara-Saika VRR of only the [e0|f0]^(m) terms the
+    targets need (ket first, then bra with the cross term; the same
+    recurrences as Methods/c_ints/two_electron_vrr.c:92-109, written in the
+    unscaled form), as named scalars so they live in registers,
+  * contraction of the m=0 targets into per-thread accumulators,
+  * HRR on contracted classes (ket then bra, Methods/c_ints/two_electron_hrr.c:82),
+  * per-Cartesian-component normalisation (Util/structures.py:850-856) and a
+    coalesced store: out[component * nq + quartet].
+
+It also emits one one-electron kernel per pair class (S, T, V; the reference's
+Methods/integrals.py:220-370 path: overlap VRR/HRR to (a|b+2), kinetic by the
+derivative formula of Methods/c_ints/one_electron_kinetic.c:60-64, nuclear
+VRR/HRR per atom).
+
+Run: python pychem_amd/csrc/gen_eri.py   (writes eri_classes.inc next to it)
+"""
+import math
+import os
+
+PAIRS = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]   # pair classes (primary, secondary)
+CLASSES = [(X, Y) for X in range(len(PAIRS)) for Y in range(X + 1)]
+
+
+def comps(l):
+    return [(lx, ly, l - lx - ly) for lx in range(l, -1, -1) for ly in range(l - lx, -1, -1)]
+
+
+def lev(c):
+    return c[0] + c[1] + c[2]
+
+
+def axis(c):
+    return 0 if c[0] else (1 if c[1] else 2)
+
+
+def dec(c, i, k=1):
+    c = list(c); c[i] -= k; return tuple(c)
+
+
+def inc(c, i):
+    c = list(c); c[i] += 1; return tuple(c)
+
+
+def nm(c):
+    return "%d%d%d" % c
+
+
+def cnorm(c):
+    return 1.0 / math.sqrt(math.gamma(c[0] + 0.5) * math.gamma(c[1] + 0.5) * math.gamma(c[2] + 0.5))
+
+
+XYZ = "xyz"
+
+
+class Emitter:
+    def __init__(self, indent="      "):
+        self.lines = []
+        self.done = set()
+        self.ind = indent
+
+    def emit(self, s):
+        self.lines.append(self.ind + s)
+
+
+def gen_eri_class(la, lb, lc, ld):
+    emax, fmax = la + lb, lc + ld
+    L = emax + fmax
+    tag = "%d%d%d%d" % (la, lb, lc, ld)
+    em = Emitter()
+
+    def vrr(e, f, m):
+        key = (e, f, m)
+        name = "v%s_%s_%d" % (nm(e), nm(f), m)
+        if key in em.done:
+            return name
+        if lev(e) == 0 and lev(f) == 0:
+            expr = "pref * F[%d]" % m
+        elif lev(e) == 0:
+            i = axis(f)
+            f1 = dec(f, i)
+            t = ["QC%s * %s" % (XYZ[i], vrr(e, f1, m)), "WQ%s * %s" % (XYZ[i], vrr(e, f1, m + 1))]
+            if f1[i] > 0:
+                f2 = dec(f1, i)
+                t.append("(real)%d * oo2q * (%s - rho_q * %s)" % (f1[i], vrr(e, f2, m), vrr(e, f2, m + 1)))
+            expr = " + ".join(t)
+        else:
+            i = axis(e)
+            e1 = dec(e, i)
+            t = ["PA%s * %s" % (XYZ[i], vrr(e1, f, m)), "WP%s * %s" % (XYZ[i], vrr(e1, f, m + 1))]
+            if e1[i] > 0:
+                e2 = dec(e1, i)
+                t.append("(real)%d * oo2p * (%s - rho_p * %s)" % (e1[i], vrr(e2, f, m), vrr(e2, f, m + 1)))
+            if f[i] > 0:
+                t.append("(real)%d * oo2pq * %s" % (f[i], vrr(e1, dec(f, i), m + 1)))
+            expr = " + ".join(t)
+        em.emit("const real %s = %s;" % (name, expr))
+        em.done.add(key)
+        return name
+
+    ecomps = [c for l in range(la, emax + 1) for c in comps(l)]
+    fcomps = [c for l in range(lc, fmax + 1) for c in comps(l)]
+    acc_names = {}
+    for e in ecomps:
+        for f in fcomps:
+            acc_names[(e, f)] = "c%s_%s" % (nm(e), nm(f))
+    # primitive-quartet body
+    for e in ecomps:
+        for f in fcomps:
+            v = vrr(e, f, 0)
+            em.emit("%s += %s;" % (acc_names[(e, f)], v))
+    body = em.lines
+
+    # HRR (post loop)
+    hr = Emitter(indent="  ")
+
+    def ket(e, c, d):
+        if lev(d) == 0:
+            return acc_names[(e, c)]
+        key = ("k", e, c, d)
+        name = "k%s_%s_%s" % (nm(e), nm(c), nm(d))
+        if key in hr.done:
+            return name
+        i = axis(d)
+        d1 = dec(d, i)
+        expr = "%s + CD%s * %s" % (ket(e, inc(c, i), d1), XYZ[i], ket(e, c, d1))
+        hr.emit("const real %s = %s;" % (name, expr))
+        hr.done.add(key)
+        return name
+
+    def bra(a, b, c, d):
+        if lev(b) == 0:
+            return ket(a, c, d)
+        key = ("b", a, b, c, d)
+        name = "h%s_%s_%s_%s" % (nm(a), nm(b), nm(c), nm(d))
+        if key in hr.done:
+            return name
+        i = axis(b)
+        b1 = dec(b, i)
+        expr = "%s + AB%s * %s" % (bra(inc(a, i), b1, c, d), XYZ[i], bra(a, b1, c, d))
+        hr.emit("const real %s = %s;" % (name, expr))
+        hr.done.add(key)
+        return name
+
+    idx = 0
+    for a in comps(la):
+        for b in comps(lb):
+            for c in comps(lc):
+                for d in comps(ld):
+                    v = bra(a, b, c, d)
+                    nrm = cnorm(a) * cnorm(b) * cnorm(c) * cnorm(d)
+                    hr.emit("out[(size_t)%d * nq + q] = (real)(%s * (real)%.17g);" % (idx, v, nrm))
+                    idx += 1
+    ncomp = idx
+
+    lines = []
+    lines.append("template <typename real>")
+    lines.append("__global__ void __launch_bounds__(ERI_BLOCK) eri_k%s(const EriArgs<real> args)" % tag)
+    lines.append("{")
+    lines.append("  const int q = blockIdx.x * blockDim.x + threadIdx.x;")
+    lines.append("  const int nq = args.nq;")
+    lines.append("  if (q >= nq) return;")
+    lines.append("  const int2 pr = args.quartets[q];")
+    lines.append("  const PairHdr bh = args.pairs[pr.x];")
+    lines.append("  const PairHdr kh = args.pairs[pr.y];")
+    lines.append("  real * __restrict__ out = args.out;")
+    for n in acc_names.values():
+        lines.append("  real %s = 0;" % n)
+    lines.append("  for (int i = 0; i < bh.n; ++i) {")
+    lines.append("    const PrimPair<real> P = args.prims[bh.off + i];")
+    lines.append("    for (int j = 0; j < kh.n; ++j) {")
+    lines.append("      const PrimPair<real> Q = args.prims[kh.off + j];")
+    lines.append("      ERI_PRIM_SETUP(%d)" % L)
+    lines.extend(body)
+    lines.append("    }")
+    lines.append("  }")
+    lines.append("  const real ABx = (real)bh.ab[0], ABy = (real)bh.ab[1], ABz = (real)bh.ab[2];")
+    lines.append("  const real CDx = (real)kh.ab[0], CDy = (real)kh.ab[1], CDz = (real)kh.ab[2];")
+    lines.append("  (void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz;")
+    lines.extend(hr.lines)
+    lines.append("}")
+    return "\n".join(lines), ncomp, len(body)
+
+
+def gen_onee(la, lb):
+    """S, T, V for one pair class; thread per shell pair."""
+    tag = "%d%d" % (la, lb)
+    es = Emitter(indent="    ")
+
+    def ovl(e):          # primitive overlap VRR, s(e) on the primary centre
+        key = ("s", e)
+        name = "s%s" % nm(e)
+        if key in es.done:
+            return name
+        if lev(e) == 0:
+            expr = "P.K"
+        else:
+            i = axis(e)
+            e1 = dec(e, i)
+            expr = "PA%s * %s" % (XYZ[i], ovl(e1))
+            if e1[i] > 0:
+                expr += " + (double)%d * P.inv2p * %s" % (e1[i], ovl(dec(e1, i)))
+        es.emit("const double %s = %s;" % (name, expr))
+        es.done.add(key)
+        return name
+
+    def ohrr(a, b):
+        if lev(b) == 0:
+            return ovl(a)
+        key = ("sh", a, b)
+        name = "sh%s_%s" % (nm(a), nm(b))
+        if key in es.done:
+            return name
+        i = axis(b)
+        b1 = dec(b, i)
+        es.emit("const double %s = %s + AB%s * %s;" % (name, ohrr(inc(a, i), b1), XYZ[i], ohrr(a, b1)))
+        es.done.add(key)
+        return name
+
+    for a in comps(la):
+        for b in comps(lb):
+            sab = ohrr(a, b)
+            t = ["beta * (double)%d * %s" % (2 * lb + 3, sab)]
+            for i in range(3):
+                t.append("- 2.0 * beta * beta * %s" % ohrr(a, tuple(b[k] + (2 if k == i else 0) for k in range(3))))
+                if b[i] > 1:
+                    t.append("- %r * %s" % (0.5 * b[i] * (b[i] - 1), ohrr(a, dec(b, i, 2))))
+            es.emit("s_%s_%s += %s;" % (nm(a), nm(b), sab))
+            es.emit("t_%s_%s += %s;" % (nm(a), nm(b), " ".join(t)))
+    sbody = es.lines
+
+    ev = Emitter(indent="      ")
+    Lv = la + lb
+
+    def nuc(e, m):
+        key = (e, m)
+        name = "n%s_%d" % (nm(e), m)
+        if key in ev.done:
+            return name
+        if lev(e) == 0:
+            expr = "vpref * F[%d]" % m
+        else:
+            i = axis(e)
+            e1 = dec(e, i)
+            expr = "PA%s * %s - PC%s * %s" % (XYZ[i], nuc(e1, m), XYZ[i], nuc(e1, m + 1))
+            if e1[i] > 0:
+                e2 = dec(e1, i)
+                expr += " + (double)%d * P.inv2p * (%s - %s)" % (e1[i], nuc(e2, m), nuc(e2, m + 1))
+        ev.emit("const double %s = %s;" % (name, expr))
+        ev.done.add(key)
+        return name
+
+    def nhrr(a, b):
+        if lev(b) == 0:
+            return nuc(a, 0)
+        key = ("vh", a, b)
+        name = "vh%s_%s" % (nm(a), nm(b))
+        if key in ev.done:
+            return name
+        i = axis(b)
+        b1 = dec(b, i)
+        ev.emit("const double %s = %s + AB%s * %s;" % (name, nhrr(inc(a, i), b1), XYZ[i], nhrr(a, b1)))
+        ev.done.add(key)
+        return name
+
+    for a in comps(la):
+        for b in comps(lb):
+            ev.emit("v_%s_%s += %s;" % (nm(a), nm(b), nhrr(a, b)))
+    vbody = ev.lines
+
+    lines = []
+    lines.append("__global__ void __launch_bounds__(128) onee_k%s(const OneeArgs args)" % tag)
+    lines.append("{")
+    lines.append("  const int t = blockIdx.x * blockDim.x + threadIdx.x;")
+    lines.append("  if (t >= args.npair) return;")
+    lines.append("  const int pidx = args.pair_list[t];")
+    lines.append("  const PairHdr h = args.pairs[pidx];")
+    lines.append("  const double ABx = h.ab[0], ABy = h.ab[1], ABz = h.ab[2];")
+    lines.append("  (void)ABx; (void)ABy; (void)ABz;")
+    for a in comps(la):
+        for b in comps(lb):
+            lines.append("  double s_%s_%s = 0, t_%s_%s = 0, v_%s_%s = 0;" % ((nm(a), nm(b)) * 3))
+    lines.append("  for (int i = 0; i < h.n; ++i) {")
+    lines.append("    const PrimPair<double> P = args.prims[h.off + i];")
+    lines.append("    const double PAx = P.PAx, PAy = P.PAy, PAz = P.PAz;")
+    lines.append("    const double beta = P.beta;")
+    lines.extend(sbody)
+    lines.append("    for (int c = 0; c < args.natom; ++c) {")
+    lines.append("      const double PCx = P.Px - args.atoms[4 * c], PCy = P.Py - args.atoms[4 * c + 1], PCz = P.Pz - args.atoms[4 * c + 2];")
+    lines.append("      const double Tn = P.p * (PCx * PCx + PCy * PCy + PCz * PCz);")
+    lines.append("      double F[%d];" % (Lv + 1))
+    lines.append("      boys_eval<%d>(Tn, F);" % Lv)
+    lines.append("      const double vpref = -args.atoms[4 * c + 3] * 1.1283791670955126 * sqrt(P.p) * P.K;")
+    lines.extend(vbody)
+    lines.append("    }")
+    lines.append("  }")
+    lines.append("  const int o1 = args.cart_off[h.s1], o2 = args.cart_off[h.s2];")
+    lines.append("  const int N = args.ncart;")
+    ia = 0
+    for a in comps(la):
+        ib = 0
+        for b in comps(lb):
+            nrm = cnorm(a) * cnorm(b)
+            lines.append("  { const size_t u = (size_t)(o1 + %d) * N + (o2 + %d), w = (size_t)(o2 + %d) * N + (o1 + %d);" % (ia, ib, ib, ia))
+            for M, v in (("S", "s"), ("T", "t"), ("V", "v")):
+                lines.append("    args.%s[u] = %s_%s_%s * %.17g; args.%s[w] = %s_%s_%s * %.17g;" %
+                             (M, v, nm(a), nm(b), nrm, M, v, nm(a), nm(b), nrm))
+            lines.append("  }")
+            ib += 1
+        ia += 1
+    lines.append("}")
+    return "\n".join(lines)
+
+
+def _write_if_changed(path, text):
+    """Keep mtimes of unchanged generated files so incremental builds skip them."""
+    if os.path.exists(path):
+        with open(path) as fh:
+            if fh.read() == text:
+                return
+    with open(path, "w") as fh:
+        fh.write(text)
+
+
+def main():
+    here = os.path.dirname(os.path.abspath(__file__))
+    gdir = os.path.join(here, "gen")
+    os.makedirs(gdir, exist_ok=True)
+    table = []
+    for X, Y in CLASSES:
+        la, lb = PAIRS[X]
+        lc, ld = PAIRS[Y]
+        tag = "%d%d%d%d" % (la, lb, lc, ld)
+        code, ncomp, nbody = gen_eri_class(la, lb, lc, ld)
+        table.append((la, lb, lc, ld, ncomp, X, Y, nbody))
+        src = ["// generated by gen_eri.py -- do not edit", '#include "../eri_common.h"', code]
+        jsrc = ["// generated by gen_eri.py -- do not edit", '#include "../jk_common.h"']
+        for rn, rt in (("d", "double"), ("f", "float")):
+            src.append("void eri_run_%s_%s(const EriArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
+            src.append("{ hipLaunchKernelGGL(eri_k%s<%s>, dim3((a.nq + ERI_BLOCK - 1) / ERI_BLOCK), dim3(ERI_BLOCK), 0, st, a); }" % (tag, rt))
+            jsrc.append("void jk_run_%s_%s(const JkArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
+            jsrc.append("{ hipLaunchKernelGGL((jk_kernel<%d, %d, %d, %d, %s>), dim3((a.nq + 127) / 128), dim3(128), 0, st, a); }" % (la, lb, lc, ld, rt))
+            jsrc.append("void scatter_run_%s_%s(const ScatterArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
+            jsrc.append("{ hipLaunchKernelGGL((scatter_kernel<%d, %d, %d, %d, %s>), dim3((a.nq + 127) / 128), dim3(128), 0, st, a); }" % (la, lb, lc, ld, rt))
+        _write_if_changed(os.path.join(gdir, "cls_%s.hip" % tag), "\n".join(src) + "\n")
+        _write_if_changed(os.path.join(gdir, "jk_%s.hip" % tag), "\n".join(jsrc) + "\n")
+    onee = ["// generated by gen_eri.py -- do not edit", '#include "../eri_common.h"']
+    for la, lb in PAIRS:
+        onee.append(gen_onee(la, lb))
+    onee.append("void onee_run(int pcls, const OneeArgs& a, hipStream_t st)")
+    onee.append("{")
+    onee.append("  const dim3 grid((a.npair + 127) / 128), block(128);")
+    onee.append("  switch (pcls) {")
+    for p, (la, lb) in enumerate(PAIRS):
+        onee.append("  case %d: hipLaunchKernelGGL(onee_k%d%d, grid, block, 0, st, a); break;" % (p, la, lb))
+    onee.append("  default: break;")
+    onee.append("  }")
+    onee.append("}")
+    _write_if_changed(os.path.join(gdir, "onee.hip"), "\n".join(onee) + "\n")
+    # dispatch header
+    d = ["// generated by gen_eri.py -- do not edit", "#pragma once", '#include "eri_common.h"',
+         "#define ERI_NCLASS %d" % len(CLASSES)]
+    for la, lb, lc, ld, ncomp, X, Y, nbody in table:
+        tag = "%d%d%d%d" % (la, lb, lc, ld)
+        for rn, rt in (("d", "double"), ("f", "float")):
+            d.append("void eri_run_%s_%s(const EriArgs<%s>&, hipStream_t);" % (tag, rn, rt))
+            d.append("void jk_run_%s_%s(const JkArgs<%s>&, hipStream_t);" % (tag, rn, rt))
+            d.append("void scatter_run_%s_%s(const ScatterArgs<%s>&, hipStream_t);" % (tag, rn, rt))
+    d.append("void onee_run(int pcls, const OneeArgs& a, hipStream_t st);")
+    d.append("// la, lb, lc, ld, components, bra pair class, ket pair class")
+    d.append("static const int eri_class_table[ERI_NCLASS][7] = {")
+    for la, lb, lc, ld, ncomp, X, Y, nbody in table:
+        d.append("  {%d, %d, %d, %d, %d, %d, %d}," % (la, lb, lc, ld, ncomp, X, Y))
+    d.append("};")
+    for kname, targ in (("eri", "EriArgs"), ("jk", "JkArgs"), ("scatter", "ScatterArgs")):
+        for rn, rt in (("d", "double"), ("f", "float")):
+            d.append("static void (*const %s_table_%s[ERI_NCLASS])(const %s<%s>&, hipStream_t) = {" % (kname, rn, targ, rt))
+            d.append("  " + ", ".join("%s_run_%d%d%d%d_%s" % (kname, t[0], t[1], t[2], t[3], rn) for t in table))
+            d.append("};")
+        d.append("template <typename real> static inline void %s_launch(int cls, const %s<real>& a, hipStream_t st)" % (kname, targ))
+        d.append("{ if (sizeof(real) == 8) %s_table_d[cls]((const %s<double>&)a, st); else %s_table_f[cls]((const %s<float>&)a, st); }" % (kname, targ, kname, targ))
+    _write_if_changed(os.path.join(here, "eri_dispatch.h"), "\n".join(d) + "\n")
+    for la, lb, lc, ld, ncomp, X, Y, nbody in table:
+        print("class (%d%d|%d%d) comps=%d body_lines=%d" % (la, lb, lc, ld, ncomp, nbody))
+    print("wrote", gdir)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,677 @@
+// pychem_hip.hip -- MI355X (gfx950) integral + Fock-build engine.
+//
+// Device side: Boys function, generated Obara-Saika/HGP ERI classes (s..d,
+// one thread per contracted shell quartet, primitive loops inside), one-
+// electron S/T/V, Schwarz bounds, J/K digestion of the stored screened ERIs,
+// dense scatter.  Host side: the C-ABI declared in include/pychem_hip.h.
+//
+// Data layout in HBM (per basis handle):
+//   pairs   PairHdr[npair]          one per shell pair a<=b, primary shell first
+//   prims   PrimPair<real>[nprimpair]  P, P-X1, p, 1/(2p), K = c1 c2 (pi/p)^1.5 e^{-ab/p |AB|^2}, beta
+//   per ERI class c: quartets int2[nq_c] (bra pair, ket pair), values real[ncomp_c][nq_c]
+//   (structure-of-arrays so that consecutive threads store / load consecutive words)
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pychem_hip.h"
+#include "eri_common.h"
+#include "eri_dispatch.h"
+
+
+__global__ void boys_test_kernel(const double* T, int n, int mmax, double* out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double F[17];
+    switch (mmax) {
+#define BCASE(M) case M: boys_eval<M>(T[i], F); break;
+        BCASE(0) BCASE(1) BCASE(2) BCASE(3) BCASE(4) BCASE(5) BCASE(6) BCASE(7) BCASE(8)
+#undef BCASE
+        default: return;
+    }
+    for (int m = 0; m <= mmax; ++m) out[(size_t)i * (mmax + 1) + m] = F[m];
+}
+
+
+// J = (J' + J'^T)/2 ; K = -(K' + K'^T)/2  (reference sign convention for K)
+__global__ void jk_finish_kernel(double* J, double* K, int n)
+{
+    const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || j < i) return;
+    const size_t u = (size_t)i * n + j, w = (size_t)j * n + i;
+    if (J) { const double s = 0.5 * (J[u] + J[w]); J[u] = s; J[w] = s; }
+    if (K) { const double s = -0.5 * (K[u] + K[w]); K[u] = s; K[w] = s; }
+}
+
+// Schwarz: per pair, max over its final-basis functions (m,n) of
+// sqrt((mn|mn)), from the Cartesian diagonal quartet block of that pair
+// (Methods/hartree_fock.py:244-254).  Spherical d shells use the rows of
+// Data/transform_basis.py:8-12 on xx,xy,xz,yy,yz,zz.
+__constant__ double c_dsph[5][6] = {
+    {0.86602540378443864676, 0, 0, -0.86602540378443864676, 0, 0},
+    {0, 1, 0, 0, 0, 0},
+    {0, 0, 1, 0, 0, 0},
+    {0, 0, 0, 0, 1, 0},
+    {-0.5, 0, 0, -0.5, 0, 1}};
+
+__global__ void schwarz_extract_kernel(const double* vals, const int* pair_ids, int nq, int n1, int n2,
+                                       const PairHdr* pairs, const int* spher, double* bound)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int nab = n1 * n2;
+    const int pid = pair_ids[q];
+    const PairHdr h = pairs[pid];
+    const int sp1 = spher ? spher[h.s1] : 0, sp2 = spher ? spher[h.s2] : 0;
+    double mx = 0.0;
+    if (!sp1 && !sp2) {
+        for (int ab = 0; ab < nab; ++ab) {
+            const double v = vals[(size_t)(ab * nab + ab) * nq + q];
+            const double s = sqrt(v > 0.0 ? v : 0.0);
+            mx = s > mx ? s : mx;
+        }
+    } else {
+        const int m1 = sp1 ? 5 : n1, m2 = sp2 ? 5 : n2;
+        for (int i = 0; i < m1; ++i)
+            for (int j = 0; j < m2; ++j) {
+                double acc = 0.0;
+                for (int a = 0; a < n1; ++a) {
+                    const double ta = sp1 ? c_dsph[i][a] : (a == i ? 1.0 : 0.0);
+                    if (ta == 0.0) continue;
+                    for (int b = 0; b < n2; ++b) {
+                        const double tb = sp2 ? c_dsph[j][b] : (b == j ? 1.0 : 0.0);
+                        if (tb == 0.0) continue;
+                        for (int c = 0; c < n1; ++c) {
+                            const double tc = sp1 ? c_dsph[i][c] : (c == i ? 1.0 : 0.0);
+                            if (tc == 0.0) continue;
+                            for (int d = 0; d < n2; ++d) {
+                                const double td = sp2 ? c_dsph[j][d] : (d == j ? 1.0 : 0.0);
+                                if (td == 0.0) continue;
+                                acc += ta * tb * tc * td * vals[(size_t)((a * n2 + b) * nab + c * n2 + d) * nq + q];
+                            }
+                        }
+                    }
+                }
+                const double s = sqrt(acc > 0.0 ? acc : 0.0);
+                mx = s > mx ? s : mx;
+            }
+    }
+    bound[pid] = mx;
+}
+
+// ==========================================================================
+// Host side
+// ==========================================================================
+static thread_local std::string g_err;
+static int fail(const std::string& m)
+{
+    g_err = m;
+    return -1;
+}
+#define HIPCHK(x)                                                                       \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if (e_ != hipSuccess) return fail(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static inline int ncart_l(int l) { return (l + 1) * (l + 2) / 2; }
+static inline int pair_class(int l1, int l2) { return l1 * (l1 + 1) / 2 + l2; }   // l1 >= l2
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t b)
+    {
+        if (b <= bytes && p) return 0;
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (b == 0) return 0;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) return fail(std::string("hipMalloc: ") + hipGetErrorString(e));
+        bytes = b;
+        return 0;
+    }
+    void release()
+    {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct EriClassStore {
+    std::vector<int2> hq;      // host copy of the quartet list
+    DevBuf q, vals;
+    int nq = 0;
+    int ncomp = 0;
+    int is32 = 0;              // values stored as float
+    long long nprimq = 0;      // primitive quartets (kept primitive pairs)
+};
+
+struct pq_basis {
+    int nshell = 0, natom = 0, ncart = 0, npair = 0, nprimpair = 0;
+    std::vector<int> l, nprim, cart_off;
+    std::vector<double> center, atoms;
+    std::vector<PairHdr> pairs;
+    std::vector<PrimPair<double>> prims;
+    std::vector<int> pair_index;            // [s1*nshell+s2] -> pair id (both orders)
+    std::vector<std::vector<int>> class_pairs;   // pair ids per pair class
+    std::vector<double> bound;              // per pair Schwarz bound
+    int have_bound = 0;
+    DevBuf d_pairs, d_prims, d_prims32, d_atoms, d_cart_off, d_bound, d_tmp_list, d_spher;
+    int any_spher = 0;
+    EriClassStore store64[ERI_NCLASS], store32[ERI_NCLASS];
+    double threshold = 1e-8, threshold32 = 0.0;
+};
+
+extern "C" const char* pq_last_error(void) { return g_err.c_str(); }
+
+extern "C" int pq_version(void) { return 1; }
+
+extern "C" int pq_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int pq_boys(int n, const double* T_dev, int mmax, double* out_dev, void* stream)
+{
+    if (mmax < 0 || mmax > 8) return fail("pq_boys: mmax must be in 0..8");
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(boys_test_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, T_dev, n, mmax,
+                       out_dev);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int pq_basis_create(int nshell, const int* l, const int* nprim, const int* prim_off, const double* center,
+                               const double* exps, const double* cc, int natom, const double* atom_xyz,
+                               const double* atom_z, pq_basis** out)
+{
+    if (!out) return fail("pq_basis_create: null output");
+    *out = nullptr;
+    if (nshell <= 0) return fail("pq_basis_create: nshell must be positive");
+    for (int s = 0; s < nshell; ++s) {
+        if (l[s] < 0 || l[s] > 2) return fail("pq_basis_create: only s, p, d shells are supported");
+        if (nprim[s] <= 0 || nprim[s] > 64) return fail("pq_basis_create: bad primitive count");
+    }
+    pq_basis* b = new pq_basis();
+    b->nshell = nshell;
+    b->natom = natom;
+    b->l.assign(l, l + nshell);
+    b->nprim.assign(nprim, nprim + nshell);
+    b->center.assign(center, center + 3 * nshell);
+    b->cart_off.resize(nshell);
+    int off = 0;
+    for (int s = 0; s < nshell; ++s) {
+        b->cart_off[s] = off;
+        off += ncart_l(l[s]);
+    }
+    b->ncart = off;
+    b->atoms.resize(4 * (size_t)std::max(natom, 1));
+    for (int a = 0; a < natom; ++a) {
+        for (int i = 0; i < 3; ++i) b->atoms[4 * a + i] = atom_xyz[3 * a + i];
+        b->atoms[4 * a + 3] = atom_z[a];
+    }
+    // shell pairs a <= b, primary = larger l (first on ties); primitive-pair
+    // quantities as Methods/c_ints/shellpair_quantities.c:28-33
+    b->pair_index.assign((size_t)nshell * nshell, -1);
+    b->class_pairs.assign(6, {});
+    const double PI = 3.14159265358979323846;
+    for (int sa = 0; sa < nshell; ++sa)
+        for (int sb = sa; sb < nshell; ++sb) {
+            PairHdr h;
+            int s1 = sa, s2 = sb;
+            if (l[sb] > l[sa]) { s1 = sb; s2 = sa; }
+            h.s1 = s1;
+            h.s2 = s2;
+            h.pcls = pair_class(l[s1], l[s2]);
+            h.pad = 0;
+            double AB[3], ab2 = 0.0;
+            for (int i = 0; i < 3; ++i) {
+                AB[i] = center[3 * s1 + i] - center[3 * s2 + i];
+                h.ab[i] = AB[i];
+                ab2 += AB[i] * AB[i];
+            }
+            h.off = (int)b->prims.size();
+            int kept = 0;
+            for (int i1 = 0; i1 < nprim[s1]; ++i1)
+                for (int i2 = 0; i2 < nprim[s2]; ++i2) {
+                    const double a1 = exps[prim_off[s1] + i1], a2 = exps[prim_off[s2] + i2];
+                    const double p = a1 + a2, sig = 1.0 / p;
+                    const double K = cc[prim_off[s1] + i1] * cc[prim_off[s2] + i2] * pow(PI * sig, 1.5) *
+                                     exp(-a1 * a2 * sig * ab2);
+                    // primitive pairs whose overlap factor underflows contribute nothing
+                    if (fabs(K) < 1e-30) continue;
+                    PrimPair<double> pp;
+                    pp.Px = (a1 * center[3 * s1] + a2 * center[3 * s2]) * sig;
+                    pp.Py = (a1 * center[3 * s1 + 1] + a2 * center[3 * s2 + 1]) * sig;
+                    pp.Pz = (a1 * center[3 * s1 + 2] + a2 * center[3 * s2 + 2]) * sig;
+                    pp.PAx = pp.Px - center[3 * s1];
+                    pp.PAy = pp.Py - center[3 * s1 + 1];
+                    pp.PAz = pp.Pz - center[3 * s1 + 2];
+                    pp.p = p;
+                    pp.inv2p = 0.5 * sig;
+                    pp.K = K;
+                    pp.beta = a2;
+                    b->prims.push_back(pp);
+                    ++kept;
+                }
+            h.n = kept;
+            const int id = (int)b->pairs.size();
+            b->pairs.push_back(h);
+            b->pair_index[(size_t)sa * nshell + sb] = id;
+            b->pair_index[(size_t)sb * nshell + sa] = id;
+            b->class_pairs[h.pcls].push_back(id);
+        }
+    b->npair = (int)b->pairs.size();
+    b->nprimpair = (int)b->prims.size();
+    std::vector<PrimPair<float>> p32(b->prims.size());
+    for (size_t i = 0; i < b->prims.size(); ++i) {
+        const PrimPair<double>& s = b->prims[i];
+        p32[i] = {(float)s.Px, (float)s.Py, (float)s.Pz, (float)s.PAx, (float)s.PAy, (float)s.PAz,
+                  (float)s.p, (float)s.inv2p, (float)s.K, (float)s.beta};
+    }
+    int rc = 0;
+    rc |= b->d_pairs.ensure(sizeof(PairHdr) * b->pairs.size());
+    rc |= b->d_prims.ensure(sizeof(PrimPair<double>) * std::max<size_t>(1, b->prims.size()));
+    rc |= b->d_prims32.ensure(sizeof(PrimPair<float>) * std::max<size_t>(1, b->prims.size()));
+    rc |= b->d_atoms.ensure(sizeof(double) * b->atoms.size());
+    rc |= b->d_cart_off.ensure(sizeof(int) * nshell);
+    rc |= b->d_bound.ensure(sizeof(double) * b->pairs.size());
+    if (rc) { delete b; return -1; }
+    hipError_t e = hipSuccess;
+    e = hipMemcpy(b->d_pairs.p, b->pairs.data(), sizeof(PairHdr) * b->pairs.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !b->prims.empty())
+        e = hipMemcpy(b->d_prims.p, b->prims.data(), sizeof(PrimPair<double>) * b->prims.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !p32.empty())
+        e = hipMemcpy(b->d_prims32.p, p32.data(), sizeof(PrimPair<float>) * p32.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_atoms.p, b->atoms.data(), sizeof(double) * b->atoms.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_cart_off.p, b->cart_off.data(), sizeof(int) * nshell, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        delete b;
+        return fail(std::string("pq_basis_create: copy failed: ") + hipGetErrorString(e));
+    }
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        b->store64[c].ncomp = b->store32[c].ncomp = eri_class_table[c][4];
+        b->store32[c].is32 = 1;
+    }
+    *out = b;
+    return 0;
+}
+
+extern "C" void pq_basis_destroy(pq_basis* b)
+{
+    if (!b) return;
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        b->store64[c].q.release(); b->store64[c].vals.release();
+        b->store32[c].q.release(); b->store32[c].vals.release();
+    }
+    b->d_pairs.release(); b->d_prims.release(); b->d_prims32.release(); b->d_atoms.release();
+    b->d_cart_off.release(); b->d_bound.release(); b->d_tmp_list.release(); b->d_spher.release();
+    delete b;
+}
+
+extern "C" int pq_basis_set_spherical(pq_basis* b, const int* spher)
+{
+    if (!b || !spher) return fail("pq_basis_set_spherical: null argument");
+    int any = 0;
+    for (int s = 0; s < b->nshell; ++s) {
+        if (spher[s] && b->l[s] != 2) return fail("pq_basis_set_spherical: only d shells have a spherical form here");
+        any |= spher[s] ? 1 : 0;
+    }
+    if (b->d_spher.ensure(sizeof(int) * b->nshell)) return -1;
+    HIPCHK(hipMemcpy(b->d_spher.p, spher, sizeof(int) * b->nshell, hipMemcpyHostToDevice));
+    b->any_spher = any;
+    b->have_bound = 0;
+    return 0;
+}
+
+extern "C" int pq_basis_info(const pq_basis* b, long long* info)
+{
+    if (!b || !info) return fail("pq_basis_info: null argument");
+    info[0] = b->ncart;
+    info[1] = b->nshell;
+    info[2] = b->npair;
+    info[3] = b->nprimpair;
+    return 0;
+}
+
+extern "C" int pq_one_electron(pq_basis* b, double* S, double* T, double* V, void* stream)
+{
+    if (!b || !S || !T || !V) return fail("pq_one_electron: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<int> all;
+    std::vector<int> offs(7, 0);
+    for (int c = 0; c < 6; ++c) {
+        offs[c] = (int)all.size();
+        all.insert(all.end(), b->class_pairs[c].begin(), b->class_pairs[c].end());
+    }
+    offs[6] = (int)all.size();
+    if (b->d_tmp_list.ensure(sizeof(int) * std::max<size_t>(1, all.size()))) return -1;
+    HIPCHK(hipMemcpyAsync(b->d_tmp_list.p, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, st));
+    for (int c = 0; c < 6; ++c) {
+        OneeArgs a;
+        a.pair_list = (const int*)b->d_tmp_list.p + offs[c];
+        a.pairs = (const PairHdr*)b->d_pairs.p;
+        a.prims = (const PrimPair<double>*)b->d_prims.p;
+        a.atoms = (const double*)b->d_atoms.p;
+        a.cart_off = (const int*)b->d_cart_off.p;
+        a.S = S; a.T = T; a.V = V;
+        a.npair = offs[c + 1] - offs[c];
+        a.natom = b->natom;
+        a.ncart = b->ncart;
+        if (a.npair > 0) onee_run(c, a, st);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+static int class_of_pair_classes(int X, int Y)
+{
+    for (int c = 0; c < ERI_NCLASS; ++c)
+        if (eri_class_table[c][5] == X && eri_class_table[c][6] == Y) return c;
+    return -1;
+}
+
+// Launch one class over a quartet list already on the device.
+template <typename real>
+static int run_class(pq_basis* b, int cls, const int2* dq, int nq, real* dout, hipStream_t st)
+{
+    if (nq <= 0) return 0;
+    EriArgs<real> a;
+    a.quartets = dq;
+    a.pairs = (const PairHdr*)b->d_pairs.p;
+    a.prims = (const PrimPair<real>*)(sizeof(real) == 8 ? b->d_prims.p : b->d_prims32.p);
+    a.out = dout;
+    a.nq = nq;
+    eri_launch<real>(cls, a, st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int pq_schwarz(pq_basis* b, double* pair_bound_host, void* stream)
+{
+    if (!b) return fail("pq_schwarz: null basis");
+    hipStream_t st = (hipStream_t)stream;
+    for (int X = 0; X < 6; ++X) {
+        const std::vector<int>& pl = b->class_pairs[X];
+        const int n = (int)pl.size();
+        if (!n) continue;
+        const int cls = class_of_pair_classes(X, X);
+        std::vector<int2> hq(n);
+        for (int i = 0; i < n; ++i) hq[i] = make_int2(pl[i], pl[i]);
+        DevBuf dq, dv, dids;
+        const int ncomp = eri_class_table[cls][4];
+        if (dq.ensure(sizeof(int2) * n) || dv.ensure(sizeof(double) * (size_t)n * ncomp) || dids.ensure(sizeof(int) * n))
+            return -1;
+        HIPCHK(hipMemcpyAsync(dq.p, hq.data(), sizeof(int2) * n, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dids.p, pl.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+        if (run_class<double>(b, cls, (const int2*)dq.p, n, (double*)dv.p, st)) return -1;
+        const int l1 = eri_class_table[cls][0], l2 = eri_class_table[cls][1];
+        hipLaunchKernelGGL(schwarz_extract_kernel, dim3((n + 127) / 128), dim3(128), 0, st, (const double*)dv.p,
+                           (const int*)dids.p, n, ncart_l(l1), ncart_l(l2), (const PairHdr*)b->d_pairs.p,
+                           b->any_spher ? (const int*)b->d_spher.p : (const int*)nullptr, (double*)b->d_bound.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        dq.release(); dv.release(); dids.release();
+    }
+    b->bound.resize(b->npair);
+    HIPCHK(hipMemcpy(b->bound.data(), b->d_bound.p, sizeof(double) * b->npair, hipMemcpyDeviceToHost));
+    b->have_bound = 1;
+    if (pair_bound_host) {
+        for (int sa = 0; sa < b->nshell; ++sa)
+            for (int sb = 0; sb < b->nshell; ++sb)
+                pair_bound_host[(size_t)sa * b->nshell + sb] = b->bound[b->pair_index[(size_t)sa * b->nshell + sb]];
+    }
+    return 0;
+}
+
+// Screened unique quartet lists: for classes X > Y all (bra, ket) with
+// Q_bra*Q_ket > thr; for X == Y the pairs ordered by descending bound and
+// ket rank >= bra rank; diagonal quartets (P|P) always kept (the reference
+// stores them from its bounds pass, Methods/hartree_fock.py:244-273).
+// Quartets with bound >= thr32 (or all if thr32 <= 0) go to the FP64 list.
+static void build_lists(pq_basis* b, double thr, double thr32)
+{
+    std::vector<std::vector<int>> sorted(6);
+    for (int X = 0; X < 6; ++X) {
+        sorted[X] = b->class_pairs[X];
+        std::stable_sort(sorted[X].begin(), sorted[X].end(),
+                         [&](int u, int v) { return b->bound[u] > b->bound[v]; });
+    }
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        b->store64[c].hq.clear(); b->store32[c].hq.clear();
+        b->store64[c].nprimq = b->store32[c].nprimq = 0;
+    }
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        const int X = eri_class_table[c][5], Y = eri_class_table[c][6];
+        const std::vector<int>& bl = sorted[X];
+        const std::vector<int>& kl = sorted[Y];
+        EriClassStore& s64 = b->store64[c];
+        EriClassStore& s32 = b->store32[c];
+        for (size_t i = 0; i < bl.size(); ++i) {
+            const int P = bl[i];
+            const double qb = b->bound[P];
+            const size_t j0 = (X == Y) ? i : 0;
+            for (size_t j = j0; j < kl.size(); ++j) {
+                const int Q = kl[j];
+                const double est = qb * b->bound[Q];
+                const bool diag = (X == Y && j == i);
+                if (!(est > thr) && !diag) break;   // kets sorted by descending bound
+                const long long np = (long long)b->pairs[P].n * b->pairs[Q].n;
+                if (thr32 > 0.0 && est < thr32 && !diag) {
+                    s32.hq.push_back(make_int2(P, Q));
+                    s32.nprimq += np;
+                } else {
+                    s64.hq.push_back(make_int2(P, Q));
+                    s64.nprimq += np;
+                }
+            }
+        }
+    }
+}
+
+extern "C" int pq_eri_build(pq_basis* b, double threshold, double fp32_threshold, void* stream)
+{
+    if (!b) return fail("pq_eri_build: null basis");
+    hipStream_t st = (hipStream_t)stream;
+    if (!b->have_bound && pq_schwarz(b, nullptr, stream)) return -1;
+    b->threshold = threshold;
+    b->threshold32 = fp32_threshold;
+    build_lists(b, threshold, fp32_threshold);
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        for (int w = 0; w < 2; ++w) {
+            EriClassStore& s = w ? b->store32[c] : b->store64[c];
+            s.nq = (int)s.hq.size();
+            if (!s.nq) continue;
+            const size_t vb = (size_t)s.nq * s.ncomp * (w ? sizeof(float) : sizeof(double));
+            if (s.q.ensure(sizeof(int2) * s.nq) || s.vals.ensure(vb)) return -1;
+            HIPCHK(hipMemcpyAsync(s.q.p, s.hq.data(), sizeof(int2) * s.nq, hipMemcpyHostToDevice, st));
+        }
+    }
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        EriClassStore& s = b->store64[c];
+        if (s.nq && run_class<double>(b, c, (const int2*)s.q.p, s.nq, (double*)s.vals.p, st)) return -1;
+        EriClassStore& t = b->store32[c];
+        if (t.nq && run_class<float>(b, c, (const int2*)t.q.p, t.nq, (float*)t.vals.p, st)) return -1;
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// Recompute the stored ERIs (same lists) -- the timed ERI-generation pass.
+extern "C" int pq_eri_recompute(pq_basis* b, void* stream)
+{
+    if (!b) return fail("pq_eri_recompute: null basis");
+    hipStream_t st = (hipStream_t)stream;
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        EriClassStore& s = b->store64[c];
+        if (s.nq && run_class<double>(b, c, (const int2*)s.q.p, s.nq, (double*)s.vals.p, st)) return -1;
+        EriClassStore& t = b->store32[c];
+        if (t.nq && run_class<float>(b, c, (const int2*)t.q.p, t.nq, (float*)t.vals.p, st)) return -1;
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int pq_eri_stats(const pq_basis* b, long long* stats, int nstats)
+{
+    // per class c: stats[c*6 + ...] = {nq64, nq32, ncomp, nprimq64, nprimq32, class id}
+    if (!b || !stats) return fail("pq_eri_stats: null argument");
+    if (nstats < ERI_NCLASS * 6) return fail("pq_eri_stats: need ERI_NCLASS*6 entries");
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        stats[c * 6 + 0] = b->store64[c].nq;
+        stats[c * 6 + 1] = b->store32[c].nq;
+        stats[c * 6 + 2] = b->store64[c].ncomp;
+        stats[c * 6 + 3] = b->store64[c].nprimq;
+        stats[c * 6 + 4] = b->store32[c].nprimq;
+        stats[c * 6 + 5] = eri_class_table[c][0] * 1000 + eri_class_table[c][1] * 100 + eri_class_table[c][2] * 10 +
+                           eri_class_table[c][3];
+    }
+    return 0;
+}
+
+extern "C" int pq_eri_nclass(void) { return ERI_NCLASS; }
+
+extern "C" int pq_jk(pq_basis* b, const double* Dj, const double* Dk, double* J, double* K, void* stream)
+{
+    if (!b || !Dj || !Dk || !J || !K) return fail("pq_jk: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int n = b->ncart;
+    HIPCHK(hipMemsetAsync(J, 0, sizeof(double) * (size_t)n * n, st));
+    HIPCHK(hipMemsetAsync(K, 0, sizeof(double) * (size_t)n * n, st));
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        for (int w = 0; w < 2; ++w) {
+            EriClassStore& s = w ? b->store32[c] : b->store64[c];
+            if (!s.nq) continue;
+            if (w) {
+                JkArgs<float> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p,
+                                (const float*)s.vals.p, Dj, Dk, J, K, s.nq, n};
+                jk_launch<float>(c, a, st);
+            } else {
+                JkArgs<double> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p,
+                                 (const double*)s.vals.p, Dj, Dk, J, K, s.nq, n};
+                jk_launch<double>(c, a, st);
+            }
+        }
+    }
+    hipLaunchKernelGGL(jk_finish_kernel, dim3((n + 127) / 128, n), dim3(128), 0, st, J, K, n);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int pq_eri_dense(pq_basis* b, double* out, void* stream)
+{
+    if (!b || !out) return fail("pq_eri_dense: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n = b->ncart;
+    HIPCHK(hipMemsetAsync(out, 0, sizeof(double) * n * n * n * n, st));
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        for (int w = 0; w < 2; ++w) {
+            EriClassStore& s = w ? b->store32[c] : b->store64[c];
+            if (!s.nq) continue;
+            if (w) {
+                ScatterArgs<float> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p,
+                                     (const float*)s.vals.p, out, s.nq, (int)n};
+                scatter_launch<float>(c, a, st);
+            } else {
+                ScatterArgs<double> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p,
+                                      (const double*)s.vals.p, out, s.nq, (int)n};
+                scatter_launch<double>(c, a, st);
+            }
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// Arbitrary shell quartets -> Cartesian normalised blocks in caller shell
+// order, packed one after another into out_host (Level-B two_electron).
+extern "C" int pq_eri_blocks(pq_basis* b, int n, const int* shells, double* out_host, void* stream)
+{
+    if (!b || (n > 0 && (!shells || !out_host))) return fail("pq_eri_blocks: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    // group by class in canonical orientation
+    struct Item { int cls, idx, P, Q, swapq; };
+    std::vector<Item> items(n);
+    std::vector<size_t> out_off(n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        const int* s = shells + 4 * i;
+        for (int k = 0; k < 4; ++k)
+            if (s[k] < 0 || s[k] >= b->nshell) return fail("pq_eri_blocks: shell index out of range");
+        int P = b->pair_index[(size_t)s[0] * b->nshell + s[1]];
+        int Q = b->pair_index[(size_t)s[2] * b->nshell + s[3]];
+        int X = b->pairs[P].pcls, Y = b->pairs[Q].pcls;
+        int swapq = 0;
+        if (Y > X) { std::swap(P, Q); std::swap(X, Y); swapq = 1; }
+        items[i] = {class_of_pair_classes(X, Y), i, P, Q, swapq};
+        out_off[i + 1] = out_off[i] + (size_t)ncart_l(b->l[s[0]]) * ncart_l(b->l[s[1]]) * ncart_l(b->l[s[2]]) *
+                                          ncart_l(b->l[s[3]]);
+    }
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        std::vector<int2> hq;
+        std::vector<int> which;
+        for (int i = 0; i < n; ++i)
+            if (items[i].cls == c) { hq.push_back(make_int2(items[i].P, items[i].Q)); which.push_back(i); }
+        if (hq.empty()) continue;
+        const int m = (int)hq.size(), ncomp = eri_class_table[c][4];
+        DevBuf dq, dv;
+        if (dq.ensure(sizeof(int2) * m) || dv.ensure(sizeof(double) * (size_t)m * ncomp)) return -1;
+        HIPCHK(hipMemcpyAsync(dq.p, hq.data(), sizeof(int2) * m, hipMemcpyHostToDevice, st));
+        if (run_class<double>(b, c, (const int2*)dq.p, m, (double*)dv.p, st)) return -1;
+        std::vector<double> hv((size_t)m * ncomp);
+        HIPCHK(hipMemcpyAsync(hv.data(), dv.p, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        const int l1 = eri_class_table[c][0], l2 = eri_class_table[c][1], l3 = eri_class_table[c][2],
+                  l4 = eri_class_table[c][3];
+        const int n1 = ncart_l(l1), n2 = ncart_l(l2), n3 = ncart_l(l3), n4 = ncart_l(l4);
+        for (int k = 0; k < m; ++k) {
+            const Item& it = items[which[k]];
+            const int* s = shells + 4 * it.idx;
+            // canonical block [c1][c2][c3][c4] with pair shells (pairs[P].s1, .s2 | pairs[Q].s1, .s2)
+            const PairHdr& hp = b->pairs[it.P];
+            const PairHdr& hq2 = b->pairs[it.Q];
+            // map caller position -> canonical slot
+            int want[4] = {s[0], s[1], s[2], s[3]};
+            int can[4] = {hp.s1, hp.s2, hq2.s1, hq2.s2};
+            // caller pair order (0,1 | 2,3); canonical may have swapped pairs and/or shells within pairs
+            int pos[4];   // pos[k] = canonical index of caller slot k
+            {
+                int cb[2] = {it.swapq ? 2 : 0, it.swapq ? 0 : 2};   // canonical pair base for caller bra/ket
+                for (int pk = 0; pk < 2; ++pk) {
+                    const int base = cb[pk];
+                    const int c0 = can[base], c1 = can[base + 1];
+                    const int w0 = want[2 * pk], w1 = want[2 * pk + 1];
+                    if (c0 == w0 && c1 == w1) { pos[2 * pk] = base; pos[2 * pk + 1] = base + 1; }
+                    else { pos[2 * pk] = base + 1; pos[2 * pk + 1] = base; }
+                    (void)c1; (void)w1;
+                }
+            }
+            const int nn[4] = {n1, n2, n3, n4};
+            const int cw[4] = {ncart_l(b->l[s[0]]), ncart_l(b->l[s[1]]), ncart_l(b->l[s[2]]), ncart_l(b->l[s[3]])};
+            double* dst = out_host + out_off[it.idx];
+            int idx[4];
+            for (idx[0] = 0; idx[0] < cw[0]; ++idx[0])
+                for (idx[1] = 0; idx[1] < cw[1]; ++idx[1])
+                    for (idx[2] = 0; idx[2] < cw[2]; ++idx[2])
+                        for (idx[3] = 0; idx[3] < cw[3]; ++idx[3]) {
+                            int ci[4];
+                            for (int t = 0; t < 4; ++t) ci[pos[t]] = idx[t];
+                            const size_t comp = ((size_t)(ci[0] * nn[1] + ci[1]) * nn[2] + ci[2]) * nn[3] + ci[3];
+                            dst[((idx[0] * cw[1] + idx[1]) * cw[2] + idx[2]) * cw[3] + idx[3]] = hv[comp * m + k];
+                        }
+        }
+    }
+    return 0;
+}

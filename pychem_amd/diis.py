@@ -1,0 +1,72 @@
+"""Pulay DIIS on the SDF - FDS residual in the orthogonal basis, FP64 on the
+host (Methods/hf_extensions/diis.py:14-155).
+
+Behaviour kept from the reference: a residual enters the subspace only while
+its max element is below 0.5 (diis.py:22,33); the oldest vectors are dropped
+while the B-matrix condition number exceeds DIIS_Max_Condition (1e7,
+diis.py:87-97; the reference's size clause never fires, diis.py:90); C1
+coefficients from solving the bordered B system (diis.py:108-112).
+"""
+import numpy as np
+
+
+class DIIS:
+    def __init__(self, threshold=0.5, max_condition=1e7, max_size=None):
+        self.threshold = threshold
+        self.max_condition = max_condition
+        self.max_size = max_size
+        self.focks = []
+        self.residuals = []
+        self.matrix = None
+        self.error = 1.0
+
+    @staticmethod
+    def residual(S, D, F, X):
+        """Xt (S D F - F D S) X, zero-padded to N x N (diis.py:48-54)."""
+        r = S @ D @ F - F @ D @ S
+        t = X.T @ r @ X
+        out = np.zeros_like(r)
+        n = t.shape[0]
+        out[:n, :n] = t
+        return out
+
+    @staticmethod
+    def _cond(m):
+        ev = np.abs(np.linalg.eigvals(m))
+        return np.inf if ev.min() == 0.0 else ev.max() / ev.min()
+
+    def extrapolate(self, res, F):
+        self.error = float(res.max())
+        if abs(res.max()) >= abs(self.threshold):
+            return F
+        self.residuals.append(res)
+        self.focks.append(F)
+        if len(self.residuals) < 2:
+            return F
+        size = len(self.residuals) + 1
+        B = np.full((size, size), -1.0)
+        B[-1, -1] = 0.0
+        if self.matrix is None:
+            for i, r1 in enumerate(self.residuals):
+                for j, r2 in enumerate(self.residuals):
+                    B[i, j] = np.vdot(r1, r2)
+        else:
+            B[:size - 1, :size - 1] = self.matrix
+            new = [np.vdot(self.residuals[-1], r) for r in self.residuals]
+            B[:len(new), size - 2] = new
+            B[size - 2, :len(new)] = new
+        self.matrix = B
+        while (self.max_size is not None and len(self.residuals) > self.max_size) or \
+                self._cond(self.matrix) > self.max_condition:
+            if len(self.residuals) == 1:
+                break
+            self.matrix = np.delete(np.delete(self.matrix, 0, 0), 0, 1)
+            self.residuals.pop(0)
+            self.focks.pop(0)
+        rhs = np.zeros(len(self.matrix))
+        rhs[-1] = -1.0
+        coeffs = np.linalg.solve(self.matrix, rhs)[:-1]
+        out = np.zeros_like(F)
+        for c, f in zip(coeffs, self.focks):
+            out += c * f
+        return out

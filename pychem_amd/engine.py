@@ -1,0 +1,151 @@
+"""Device-resident integral engine: one basis handle per molecule.
+
+Wraps the C ABI (include/pychem_hip.h).  torch is used only for device
+memory and the current HIP stream; every integral, J/K and transform runs in
+the library's own kernels.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, dptr, iptr
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Engine:
+    """Integrals for one molecule on one GPU (Cartesian AO basis internally)."""
+
+    def __init__(self, mol, device="cuda"):
+        if not torch.cuda.is_available():
+            raise _lib.HipError("no GPU visible: the HIP engine has no CPU fallback")
+        self.mol = mol
+        self.device = torch.device(device)
+        t = mol.shell_table()
+        self.table = t
+        self._keep = [np.ascontiguousarray(t[k], dtype=np.int32) for k in ("l", "nprim", "prim_off")]
+        self._keepd = [np.ascontiguousarray(t[k], dtype=float) for k in ("center", "exps", "cc", "atom_xyz", "atom_z")]
+        h = ctypes.c_void_p()
+        L = _lib.lib()
+        check(L.pq_basis_create(t["nshell"], iptr(self._keep[0]), iptr(self._keep[1]), iptr(self._keep[2]),
+                                dptr(self._keepd[0]), dptr(self._keepd[1]), dptr(self._keepd[2]),
+                                t["natom"], dptr(self._keepd[3]), dptr(self._keepd[4]), ctypes.byref(h)),
+              "pq_basis_create")
+        self.h = h
+        info = np.zeros(4, dtype=np.int64)
+        check(L.pq_basis_info(h, info.ctypes.data_as(_lib.P_ll)), "pq_basis_info")
+        self.ncart, self.nshell, self.npair, self.nprimpair = (int(x) for x in info)
+        self.nao = t["nao"]
+        self.cartesian = self.nao == self.ncart
+        self.T = None if self.cartesian else torch.tensor(mol.cart_to_final(), device=self.device)
+        if not self.cartesian:
+            self._spher = np.ascontiguousarray(t["spher"], dtype=np.int32)
+            check(L.pq_basis_set_spherical(h, iptr(self._spher)), "pq_basis_set_spherical")
+        self.eri_built = False
+        self.threshold = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().pq_basis_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- basis transforms ------------------------------------------------
+    def to_final(self, M):
+        return M if self.cartesian else self.T @ M @ self.T.T
+
+    def to_cart(self, D):
+        return D if self.cartesian else self.T.T @ D @ self.T
+
+    # -- integrals -------------------------------------------------------
+    def one_electron(self):
+        """(S, T, V) in the final basis, torch float64 on the device."""
+        n = self.ncart
+        S = torch.empty((n, n), dtype=torch.float64, device=self.device)
+        T = torch.empty_like(S)
+        V = torch.empty_like(S)
+        check(_lib.lib().pq_one_electron(self.h, _ptr(S), _ptr(T), _ptr(V), _stream()), "pq_one_electron")
+        return self.to_final(S), self.to_final(T), self.to_final(V)
+
+    def schwarz(self):
+        out = np.zeros((self.nshell, self.nshell))
+        check(_lib.lib().pq_schwarz(self.h, dptr(out), _stream()), "pq_schwarz")
+        return out
+
+    def build_eri(self, threshold=1e-8, fp32_threshold=0.0):
+        check(_lib.lib().pq_eri_build(self.h, float(threshold), float(fp32_threshold), _stream()), "pq_eri_build")
+        self.eri_built = True
+        self.threshold = threshold
+        return self.stats()
+
+    def recompute_eri(self):
+        check(_lib.lib().pq_eri_recompute(self.h, _stream()), "pq_eri_recompute")
+
+    def stats(self):
+        nc = _lib.lib().pq_eri_nclass()
+        st = np.zeros(nc * 6, dtype=np.int64)
+        check(_lib.lib().pq_eri_stats(self.h, st.ctypes.data_as(_lib.P_ll), nc * 6), "pq_eri_stats")
+        st = st.reshape(nc, 6)
+        return [dict(cls="%04d" % r[5], nq64=int(r[0]), nq32=int(r[1]), ncomp=int(r[2]),
+                     nprim64=int(r[3]), nprim32=int(r[4])) for r in st]
+
+    def jk(self, Dj, Dk):
+        """J (from Dj) and K (from Dk, with the reference's minus sign), final basis."""
+        if not self.eri_built:
+            raise _lib.HipError("build_eri() first")
+        Dj = self.to_cart(Dj).contiguous()
+        Dk = Dj if Dk is None else self.to_cart(Dk).contiguous()
+        n = self.ncart
+        J = torch.empty((n, n), dtype=torch.float64, device=self.device)
+        K = torch.empty_like(J)
+        check(_lib.lib().pq_jk(self.h, _ptr(Dj), _ptr(Dk), _ptr(J), _ptr(K), _stream()), "pq_jk")
+        return self.to_final(J), self.to_final(K)
+
+    def eri_dense(self):
+        """Dense screened (ab|cd) [nao]^4 in the final basis (device)."""
+        n = self.ncart
+        out = torch.empty((n, n, n, n), dtype=torch.float64, device=self.device)
+        check(_lib.lib().pq_eri_dense(self.h, _ptr(out), _stream()), "pq_eri_dense")
+        if self.cartesian:
+            return out
+        T = self.T
+        out = torch.einsum("ia,abcd->ibcd", T, out)
+        out = torch.einsum("jb,ibcd->ijcd", T, out)
+        out = torch.einsum("kc,ijcd->ijkd", T, out)
+        return torch.einsum("ld,ijkd->ijkl", T, out)
+
+    def eri_blocks(self, quartets):
+        """Cartesian normalised blocks for shell quartets [(a,b,c,d), ...]."""
+        q = np.ascontiguousarray(np.asarray(quartets, dtype=np.int32).reshape(-1, 4))
+        l = self.table["l"]
+        nc = (l + 1) * (l + 2) // 2
+        sizes = [int(nc[a] * nc[b] * nc[c] * nc[d]) for a, b, c, d in q]
+        out = np.zeros(sum(sizes))
+        check(_lib.lib().pq_eri_blocks(self.h, len(q), iptr(q), dptr(out), _stream()), "pq_eri_blocks")
+        res, off = [], 0
+        for (a, b, c, d), s in zip(q, sizes):
+            res.append(out[off:off + s].reshape(nc[a], nc[b], nc[c], nc[d]))
+            off += s
+        return res
+
+
+def boys(T, mmax):
+    """Device Boys function (test entry point): returns numpy [n][mmax+1]."""
+    Tt = torch.as_tensor(np.asarray(T, dtype=float).reshape(-1), device="cuda")
+    out = torch.empty((Tt.numel(), mmax + 1), dtype=torch.float64, device="cuda")
+    check(_lib.lib().pq_boys(Tt.numel(), _ptr(Tt), mmax, _ptr(out), _stream()), "pq_boys")
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
