@@ -70,8 +70,20 @@ int pq_schwarz(pq_basis* b, double* pair_bound_host, void* stream);
  * 157-378).  Quartets whose bound product is below fp32_threshold are
  * computed and stored in FP32 (0 disables the split). */
 int pq_eri_build(pq_basis* b, double threshold, double fp32_threshold, void* stream);
-/* Recompute the stored ERIs over the same quartet lists. */
+/* pq_eri_build in two steps: plan (Schwarz if needed, screened class lists,
+ * HBM allocation) and compute (all class kernels over the planned lists). */
+int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold, void* stream);
+int pq_eri_compute(pq_basis* b, void* stream);
+/* Recompute the stored ERIs over the same quartet lists (= pq_eri_compute). */
 int pq_eri_recompute(pq_basis* b, void* stream);
+/* Live kernel timing with hipEvents on the caller's stream (on != 0). */
+int pq_set_timing(pq_basis* b, int on);
+/* ms of the last timed ERI pass per class, eri_ms[2*nclass] (slot 2c: FP64
+ * list, 2c+1: FP32 list, -1 if not launched), and of the last pq_jk call. */
+int pq_get_timings(pq_basis* b, double* eri_ms, double* jk_ms);
+/* Recompute one class of the stored ERIs: cls in [0, pq_eri_nclass()),
+ * w = 0 for the FP64 list, 1 for the FP32 list. */
+int pq_eri_run_class(pq_basis* b, int cls, int w, void* stream);
 /* Number of ERI classes (21 for s..d). */
 int pq_eri_nclass(void);
 /* Per class c: stats[6c..6c+5] = nq_fp64, nq_fp32, components per quartet,

@@ -43,9 +43,11 @@ def _residual(S, D, F, X):
     return out
 
 
-def _diis_step(res, F, st, threshold=0.5, max_cond=1e7):
-    """Methods/hf_extensions/diis.py:32-41 with C1 coefficients."""
-    if abs(res.max()) < abs(threshold):
+def _diis_step(res, F, st, threshold=0.5, max_cond=1e7, max_size=None):
+    """Methods/hf_extensions/diis.py:32-41 with C1 coefficients.  The
+    defaults are the reference's; threshold=inf, max_cond=1e30, max_size=8
+    give plain Pulay DIIS (the reference's size clause never fires)."""
+    if abs(res.max()) < abs(threshold) and (threshold < 1 or np.abs(res).max() > 1e-14):
         st.res.append(res)
         st.focks.append(F)
         if len(st.res) > 1:
@@ -68,7 +70,7 @@ def _diis_step(res, F, st, threshold=0.5, max_cond=1e7):
                 ev = np.abs(np.linalg.eigvals(m))
                 return np.inf if ev.min() == 0 else ev.max() / ev.min()
             c = cond(st.matrix)
-            while c > max_cond:
+            while c > max_cond or (max_size is not None and len(st.res) > max_size):
                 if len(st.res) == 1:
                     break
                 st.matrix = np.delete(np.delete(st.matrix, 0, 0), 0, 1)
@@ -77,13 +79,16 @@ def _diis_step(res, F, st, threshold=0.5, max_cond=1e7):
                 c = cond(st.matrix)
             rhs = np.zeros(len(st.matrix))
             rhs[-1] = -1.0
-            coeffs = np.linalg.solve(st.matrix, rhs)[:-1]
+            try:
+                coeffs = np.linalg.solve(st.matrix, rhs)[:-1]
+            except np.linalg.LinAlgError:
+                coeffs = np.linalg.lstsq(st.matrix, rhs, rcond=None)[0][:-1]
             F = sum(c * f for c, f in zip(coeffs, st.focks))
     return F
 
 
 def scf(H, S, eri, nalpha, nbeta, enuc, conv=1e-7, maxiter=50, use_diis=True, guess=None,
-        jk=None):
+        jk=None, diis=None):
     """UHF/RHF SCF exactly as the reference loop.  ``jk(Dt, Da, Db)`` may be
     supplied to replace the dense einsums; it returns (J, Ka, Kb) with the
     reference's sign convention (K negative)."""
@@ -132,8 +137,8 @@ def scf(H, S, eri, nalpha, nbeta, enuc, conv=1e-7, maxiter=50, use_diis=True, gu
         if use_diis and it > 1:
             ra = _residual(S, Da, Fa, X)
             rb = _residual(S, Db, Fb, X)
-            Fa = _diis_step(ra, Fa, da)
-            Fb = _diis_step(rb, Fb, db)
+            Fa = _diis_step(ra, Fa, da, **(diis or {}))
+            Fb = _diis_step(rb, Fb, db, **(diis or {}))
         ea, Ca = make_mos(Fa)
         eb, Cb = make_mos(Fb)
         ea, Ca = sort_occ(ea, Ca, nalpha)
@@ -160,12 +165,15 @@ def nuclear_repulsion(mol):
     return e
 
 
-def run_rhf(mol, conv=1e-10, maxiter=100, thresh=1e-8):
+PULAY = dict(threshold=float("inf"), max_cond=1e14, max_size=8)
+
+
+def run_rhf(mol, conv=1e-10, maxiter=100, thresh=1e-8, diis=None):
     S, T, V = cint.one_electron(mol)
     H = T + V
     eri, nq = cint.eri_dense(mol, thresh)
     res = scf(H, S, eri, mol.NAlphaElectrons, mol.NBetaElectrons, nuclear_repulsion(mol),
-              conv=conv, maxiter=maxiter)
+              conv=conv, maxiter=maxiter, diis=diis)
     res.update(H=H, S=S, eri=eri, nquartets=nq)
     return res
 

@@ -76,7 +76,11 @@ def gen_eri_class(la, lb, lc, ld):
     tag = "%d%d%d%d" % (la, lb, lc, ld)
     em = Emitter()
 
+    flops = {"vrr": 0, "contract": 0, "hrr": 0}
+
     def vrr(e, f, m):
+        # flops priced as the reference's two_electron_vrr.c:92-109: 6 for the
+        # base terms, +5 with the a-term, +4 with the cross term
         key = (e, f, m)
         name = "v%s_%s_%d" % (nm(e), nm(f), m)
         if key in em.done:
@@ -87,19 +91,24 @@ def gen_eri_class(la, lb, lc, ld):
             i = axis(f)
             f1 = dec(f, i)
             t = ["QC%s * %s" % (XYZ[i], vrr(e, f1, m)), "WQ%s * %s" % (XYZ[i], vrr(e, f1, m + 1))]
+            flops["vrr"] += 6
             if f1[i] > 0:
                 f2 = dec(f1, i)
                 t.append("(real)%d * oo2q * (%s - rho_q * %s)" % (f1[i], vrr(e, f2, m), vrr(e, f2, m + 1)))
+                flops["vrr"] += 5
             expr = " + ".join(t)
         else:
             i = axis(e)
             e1 = dec(e, i)
             t = ["PA%s * %s" % (XYZ[i], vrr(e1, f, m)), "WP%s * %s" % (XYZ[i], vrr(e1, f, m + 1))]
+            flops["vrr"] += 6
             if e1[i] > 0:
                 e2 = dec(e1, i)
                 t.append("(real)%d * oo2p * (%s - rho_p * %s)" % (e1[i], vrr(e2, f, m), vrr(e2, f, m + 1)))
+                flops["vrr"] += 5
             if f[i] > 0:
                 t.append("(real)%d * oo2pq * %s" % (f[i], vrr(e1, dec(f, i), m + 1)))
+                flops["vrr"] += 4
             expr = " + ".join(t)
         em.emit("const real %s = %s;" % (name, expr))
         em.done.add(key)
@@ -116,6 +125,7 @@ def gen_eri_class(la, lb, lc, ld):
         for f in fcomps:
             v = vrr(e, f, 0)
             em.emit("%s += %s;" % (acc_names[(e, f)], v))
+            flops["contract"] += 3   # two_electron_contract.c:45
     body = em.lines
 
     # HRR (post loop)
@@ -131,6 +141,7 @@ def gen_eri_class(la, lb, lc, ld):
         i = axis(d)
         d1 = dec(d, i)
         expr = "%s + CD%s * %s" % (ket(e, inc(c, i), d1), XYZ[i], ket(e, c, d1))
+        flops["hrr"] += 2   # two_electron_hrr.c:82
         hr.emit("const real %s = %s;" % (name, expr))
         hr.done.add(key)
         return name
@@ -145,6 +156,7 @@ def gen_eri_class(la, lb, lc, ld):
         i = axis(b)
         b1 = dec(b, i)
         expr = "%s + AB%s * %s" % (bra(inc(a, i), b1, c, d), XYZ[i], bra(a, b1, c, d))
+        flops["hrr"] += 2
         hr.emit("const real %s = %s;" % (name, expr))
         hr.done.add(key)
         return name
@@ -186,6 +198,7 @@ def gen_eri_class(la, lb, lc, ld):
     lines.append("  (void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz;")
     lines.extend(hr.lines)
     lines.append("}")
+    gen_eri_class.flops = dict(flops, boys=L + 1)
     return "\n".join(lines), ncomp, len(body)
 
 
@@ -334,12 +347,14 @@ def main():
     gdir = os.path.join(here, "gen")
     os.makedirs(gdir, exist_ok=True)
     table = []
+    flop_table = {}
     for X, Y in CLASSES:
         la, lb = PAIRS[X]
         lc, ld = PAIRS[Y]
         tag = "%d%d%d%d" % (la, lb, lc, ld)
         code, ncomp, nbody = gen_eri_class(la, lb, lc, ld)
         table.append((la, lb, lc, ld, ncomp, X, Y, nbody))
+        flop_table[tag] = gen_eri_class.flops
         src = ["// generated by gen_eri.py -- do not edit", '#include "../eri_common.h"', code]
         jsrc = ["// generated by gen_eri.py -- do not edit", '#include "../jk_common.h"']
         for rn, rt in (("d", "double"), ("f", "float")):
@@ -347,6 +362,9 @@ def main():
             src.append("{ hipLaunchKernelGGL(eri_k%s<%s>, dim3((a.nq + ERI_BLOCK - 1) / ERI_BLOCK), dim3(ERI_BLOCK), 0, st, a); }" % (tag, rt))
             jsrc.append("void jk_run_%s_%s(const JkArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
             jsrc.append("{ hipLaunchKernelGGL((jk_kernel<%d, %d, %d, %d, %s>), dim3((a.nq + 127) / 128), dim3(128), 0, st, a); }" % (la, lb, lc, ld, rt))
+            jsrc.append("void jkrun_run_%s_%s(const JkRunArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
+            jsrc.append("{ hipLaunchKernelGGL((jk_run_kernel<%d, %d, %d, %d, %s>), dim3(a.nitems), dim3(64), "
+                        "sizeof(double) * %d * a.n, st, a); }" % (la, lb, lc, ld, rt, (la + 1) * (la + 2) // 2 + (lb + 1) * (lb + 2) // 2))
             jsrc.append("void scatter_run_%s_%s(const ScatterArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
             jsrc.append("{ hipLaunchKernelGGL((scatter_kernel<%d, %d, %d, %d, %s>), dim3((a.nq + 127) / 128), dim3(128), 0, st, a); }" % (la, lb, lc, ld, rt))
         _write_if_changed(os.path.join(gdir, "cls_%s.hip" % tag), "\n".join(src) + "\n")
@@ -365,13 +383,14 @@ def main():
     onee.append("}")
     _write_if_changed(os.path.join(gdir, "onee.hip"), "\n".join(onee) + "\n")
     # dispatch header
-    d = ["// generated by gen_eri.py -- do not edit", "#pragma once", '#include "eri_common.h"',
+    d = ["// generated by gen_eri.py -- do not edit", "#pragma once", '#include "jk_common.h"',
          "#define ERI_NCLASS %d" % len(CLASSES)]
     for la, lb, lc, ld, ncomp, X, Y, nbody in table:
         tag = "%d%d%d%d" % (la, lb, lc, ld)
         for rn, rt in (("d", "double"), ("f", "float")):
             d.append("void eri_run_%s_%s(const EriArgs<%s>&, hipStream_t);" % (tag, rn, rt))
             d.append("void jk_run_%s_%s(const JkArgs<%s>&, hipStream_t);" % (tag, rn, rt))
+            d.append("void jkrun_run_%s_%s(const JkRunArgs<%s>&, hipStream_t);" % (tag, rn, rt))
             d.append("void scatter_run_%s_%s(const ScatterArgs<%s>&, hipStream_t);" % (tag, rn, rt))
     d.append("void onee_run(int pcls, const OneeArgs& a, hipStream_t st);")
     d.append("// la, lb, lc, ld, components, bra pair class, ket pair class")
@@ -379,7 +398,7 @@ def main():
     for la, lb, lc, ld, ncomp, X, Y, nbody in table:
         d.append("  {%d, %d, %d, %d, %d, %d, %d}," % (la, lb, lc, ld, ncomp, X, Y))
     d.append("};")
-    for kname, targ in (("eri", "EriArgs"), ("jk", "JkArgs"), ("scatter", "ScatterArgs")):
+    for kname, targ in (("eri", "EriArgs"), ("jk", "JkArgs"), ("jkrun", "JkRunArgs"), ("scatter", "ScatterArgs")):
         for rn, rt in (("d", "double"), ("f", "float")):
             d.append("static void (*const %s_table_%s[ERI_NCLASS])(const %s<%s>&, hipStream_t) = {" % (kname, rn, targ, rt))
             d.append("  " + ", ".join("%s_run_%d%d%d%d_%s" % (kname, t[0], t[1], t[2], t[3], rn) for t in table))
@@ -387,6 +406,8 @@ def main():
         d.append("template <typename real> static inline void %s_launch(int cls, const %s<real>& a, hipStream_t st)" % (kname, targ))
         d.append("{ if (sizeof(real) == 8) %s_table_d[cls]((const %s<double>&)a, st); else %s_table_f[cls]((const %s<float>&)a, st); }" % (kname, targ, kname, targ))
     _write_if_changed(os.path.join(here, "eri_dispatch.h"), "\n".join(d) + "\n")
+    import json
+    _write_if_changed(os.path.join(here, "eri_flops.json"), json.dumps(flop_table, indent=1, sort_keys=True) + "\n")
     for la, lb, lc, ld, ncomp, X, Y, nbody in table:
         print("class (%d%d|%d%d) comps=%d body_lines=%d" % (la, lb, lc, ld, ncomp, nbody))
     print("wrote", gdir)

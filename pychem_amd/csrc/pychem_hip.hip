@@ -149,7 +149,9 @@ struct DevBuf {
 
 struct EriClassStore {
     std::vector<int2> hq;      // host copy of the quartet list
-    DevBuf q, vals;
+    std::vector<int2> hitems;  // J/K work items: [start, end) ranges with one bra pair
+    DevBuf q, vals, items;
+    int nitems = 0;
     int nq = 0;
     int ncomp = 0;
     int is32 = 0;              // values stored as float
@@ -170,7 +172,23 @@ struct pq_basis {
     int any_spher = 0;
     EriClassStore store64[ERI_NCLASS], store32[ERI_NCLASS];
     double threshold = 1e-8, threshold32 = 0.0;
+    // optional live timing (hipEvents on the caller's stream)
+    int timing = 0;
+    hipEvent_t ev_cls[2 * ERI_NCLASS][2] = {};
+    int ev_cls_used[2 * ERI_NCLASS] = {};
+    hipEvent_t ev_jk[2] = {};
+    int ev_jk_used = 0;
 };
+
+static int ensure_events(pq_basis* b)
+{
+    if (b->ev_jk[0]) return 0;
+    for (int i = 0; i < 2 * ERI_NCLASS; ++i)
+        for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreate(&b->ev_cls[i][k]));
+    HIPCHK(hipEventCreate(&b->ev_jk[0]));
+    HIPCHK(hipEventCreate(&b->ev_jk[1]));
+    return 0;
+}
 
 extern "C" const char* pq_last_error(void) { return g_err.c_str(); }
 
@@ -313,11 +331,17 @@ extern "C" void pq_basis_destroy(pq_basis* b)
 {
     if (!b) return;
     for (int c = 0; c < ERI_NCLASS; ++c) {
-        b->store64[c].q.release(); b->store64[c].vals.release();
-        b->store32[c].q.release(); b->store32[c].vals.release();
+        b->store64[c].q.release(); b->store64[c].vals.release(); b->store64[c].items.release();
+        b->store32[c].q.release(); b->store32[c].vals.release(); b->store32[c].items.release();
     }
     b->d_pairs.release(); b->d_prims.release(); b->d_prims32.release(); b->d_atoms.release();
     b->d_cart_off.release(); b->d_bound.release(); b->d_tmp_list.release(); b->d_spher.release();
+    if (b->ev_jk[0]) {
+        for (int i = 0; i < 2 * ERI_NCLASS; ++i)
+            for (int k = 0; k < 2; ++k) (void)hipEventDestroy(b->ev_cls[i][k]);
+        (void)hipEventDestroy(b->ev_jk[0]);
+        (void)hipEventDestroy(b->ev_jk[1]);
+    }
     delete b;
 }
 
@@ -482,9 +506,9 @@ static void build_lists(pq_basis* b, double thr, double thr32)
     }
 }
 
-extern "C" int pq_eri_build(pq_basis* b, double threshold, double fp32_threshold, void* stream)
+extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold, void* stream)
 {
-    if (!b) return fail("pq_eri_build: null basis");
+    if (!b) return fail("pq_eri_plan: null basis");
     hipStream_t st = (hipStream_t)stream;
     if (!b->have_bound && pq_schwarz(b, nullptr, stream)) return -1;
     b->threshold = threshold;
@@ -498,31 +522,109 @@ extern "C" int pq_eri_build(pq_basis* b, double threshold, double fp32_threshold
             const size_t vb = (size_t)s.nq * s.ncomp * (w ? sizeof(float) : sizeof(double));
             if (s.q.ensure(sizeof(int2) * s.nq) || s.vals.ensure(vb)) return -1;
             HIPCHK(hipMemcpyAsync(s.q.p, s.hq.data(), sizeof(int2) * s.nq, hipMemcpyHostToDevice, st));
+            // J/K work items: runs of equal bra pair, cut into pieces of at most
+            // JK_RUN quartets so that long runs spread over several waves; the
+            // piece length shrinks for classes with few bra pairs so that every
+            // class still launches ~2048 waves (8 per CU)
+            int JK_RUN = s.nq / 2048;
+            JK_RUN = JK_RUN < 64 ? 64 : (JK_RUN > 1024 ? 1024 : JK_RUN);
+            s.hitems.clear();
+            int start = 0;
+            for (int i = 1; i <= s.nq; ++i) {
+                if (i == s.nq || s.hq[i].x != s.hq[start].x || i - start == JK_RUN) {
+                    s.hitems.push_back(make_int2(start, i));
+                    start = i;
+                }
+            }
+            s.nitems = (int)s.hitems.size();
+            if (s.items.ensure(sizeof(int2) * s.nitems)) return -1;
+            HIPCHK(hipMemcpyAsync(s.items.p, s.hitems.data(), sizeof(int2) * s.nitems, hipMemcpyHostToDevice, st));
         }
     }
+    return 0;
+}
+
+// All class kernels over the planned lists (the ERI-generation pass).
+extern "C" int pq_eri_compute(pq_basis* b, void* stream)
+{
+    if (!b) return fail("pq_eri_compute: null basis");
+    hipStream_t st = (hipStream_t)stream;
+    if (b->timing && ensure_events(b)) return -1;
     for (int c = 0; c < ERI_NCLASS; ++c) {
-        EriClassStore& s = b->store64[c];
-        if (s.nq && run_class<double>(b, c, (const int2*)s.q.p, s.nq, (double*)s.vals.p, st)) return -1;
-        EriClassStore& t = b->store32[c];
-        if (t.nq && run_class<float>(b, c, (const int2*)t.q.p, t.nq, (float*)t.vals.p, st)) return -1;
+        for (int w = 0; w < 2; ++w) {
+            EriClassStore& s = w ? b->store32[c] : b->store64[c];
+            const int slot = 2 * c + w;
+            b->ev_cls_used[slot] = 0;
+            if (!s.nq) continue;
+            if (b->timing) HIPCHK(hipEventRecord(b->ev_cls[slot][0], st));
+            int rc = w ? run_class<float>(b, c, (const int2*)s.q.p, s.nq, (float*)s.vals.p, st)
+                       : run_class<double>(b, c, (const int2*)s.q.p, s.nq, (double*)s.vals.p, st);
+            if (rc) return -1;
+            if (b->timing) {
+                HIPCHK(hipEventRecord(b->ev_cls[slot][1], st));
+                b->ev_cls_used[slot] = 1;
+            }
+        }
     }
     HIPCHK(hipGetLastError());
     return 0;
 }
 
-// Recompute the stored ERIs (same lists) -- the timed ERI-generation pass.
-extern "C" int pq_eri_recompute(pq_basis* b, void* stream)
+extern "C" int pq_eri_build(pq_basis* b, double threshold, double fp32_threshold, void* stream)
 {
-    if (!b) return fail("pq_eri_recompute: null basis");
-    hipStream_t st = (hipStream_t)stream;
-    for (int c = 0; c < ERI_NCLASS; ++c) {
-        EriClassStore& s = b->store64[c];
-        if (s.nq && run_class<double>(b, c, (const int2*)s.q.p, s.nq, (double*)s.vals.p, st)) return -1;
-        EriClassStore& t = b->store32[c];
-        if (t.nq && run_class<float>(b, c, (const int2*)t.q.p, t.nq, (float*)t.vals.p, st)) return -1;
-    }
-    HIPCHK(hipGetLastError());
+    if (pq_eri_plan(b, threshold, fp32_threshold, stream)) return -1;
+    return pq_eri_compute(b, stream);
+}
+
+extern "C" int pq_eri_recompute(pq_basis* b, void* stream) { return pq_eri_compute(b, stream); }
+
+extern "C" int pq_set_timing(pq_basis* b, int on)
+{
+    if (!b) return fail("pq_set_timing: null basis");
+    b->timing = on ? 1 : 0;
+    if (b->timing) return ensure_events(b);
     return 0;
+}
+
+// Elapsed ms of the last timed ERI pass per class (slot 2c = FP64 list,
+// 2c+1 = FP32 list; -1 when not launched) and of the last pq_jk call.
+extern "C" int pq_get_timings(pq_basis* b, double* eri_ms, double* jk_ms)
+{
+    if (!b) return fail("pq_get_timings: null basis");
+    if (!b->timing) return fail("pq_get_timings: timing not enabled");
+    if (eri_ms)
+        for (int i = 0; i < 2 * ERI_NCLASS; ++i) {
+            eri_ms[i] = -1.0;
+            if (!b->ev_cls_used[i]) continue;
+            HIPCHK(hipEventSynchronize(b->ev_cls[i][1]));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, b->ev_cls[i][0], b->ev_cls[i][1]));
+            eri_ms[i] = ms;
+        }
+    if (jk_ms) {
+        *jk_ms = -1.0;
+        if (b->ev_jk_used) {
+            HIPCHK(hipEventSynchronize(b->ev_jk[1]));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, b->ev_jk[0], b->ev_jk[1]));
+            *jk_ms = ms;
+        }
+    }
+    return 0;
+}
+
+// Recompute one stored class (w = 0: FP64 list, 1: FP32 list).
+extern "C" int pq_eri_run_class(pq_basis* b, int cls, int w, void* stream)
+{
+    if (!b) return fail("pq_eri_run_class: null basis");
+    if (cls < 0 || cls >= ERI_NCLASS || w < 0 || w > 1) return fail("pq_eri_run_class: bad class");
+    hipStream_t st = (hipStream_t)stream;
+    if (w) {
+        EriClassStore& t = b->store32[cls];
+        return t.nq ? run_class<float>(b, cls, (const int2*)t.q.p, t.nq, (float*)t.vals.p, st) : 0;
+    }
+    EriClassStore& s = b->store64[cls];
+    return s.nq ? run_class<double>(b, cls, (const int2*)s.q.p, s.nq, (double*)s.vals.p, st) : 0;
 }
 
 extern "C" int pq_eri_stats(const pq_basis* b, long long* stats, int nstats)
@@ -549,12 +651,32 @@ extern "C" int pq_jk(pq_basis* b, const double* Dj, const double* Dk, double* J,
     if (!b || !Dj || !Dk || !J || !K) return fail("pq_jk: null argument");
     hipStream_t st = (hipStream_t)stream;
     const int n = b->ncart;
+    if (b->timing) {
+        if (ensure_events(b)) return -1;
+        HIPCHK(hipEventRecord(b->ev_jk[0], st));
+    }
     HIPCHK(hipMemsetAsync(J, 0, sizeof(double) * (size_t)n * n, st));
     HIPCHK(hipMemsetAsync(K, 0, sizeof(double) * (size_t)n * n, st));
     for (int c = 0; c < ERI_NCLASS; ++c) {
+        const int nbra = ncart_l(eri_class_table[c][0]) + ncart_l(eri_class_table[c][1]);
+        const bool use_runs = sizeof(double) * (size_t)nbra * n <= 65536;
         for (int w = 0; w < 2; ++w) {
             EriClassStore& s = w ? b->store32[c] : b->store64[c];
             if (!s.nq) continue;
+            if (use_runs) {
+                if (w) {
+                    JkRunArgs<float> a{(const int2*)s.q.p, (const int2*)s.items.p, (const PairHdr*)b->d_pairs.p,
+                                       (const int*)b->d_cart_off.p, (const float*)s.vals.p, Dj, Dk, J, K, s.nq, n,
+                                       s.nitems};
+                    jkrun_launch<float>(c, a, st);
+                } else {
+                    JkRunArgs<double> a{(const int2*)s.q.p, (const int2*)s.items.p, (const PairHdr*)b->d_pairs.p,
+                                        (const int*)b->d_cart_off.p, (const double*)s.vals.p, Dj, Dk, J, K, s.nq, n,
+                                        s.nitems};
+                    jkrun_launch<double>(c, a, st);
+                }
+                continue;
+            }
             if (w) {
                 JkArgs<float> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p,
                                 (const float*)s.vals.p, Dj, Dk, J, K, s.nq, n};
@@ -568,6 +690,10 @@ extern "C" int pq_jk(pq_basis* b, const double* Dj, const double* Dk, double* J,
     }
     hipLaunchKernelGGL(jk_finish_kernel, dim3((n + 127) / 128, n), dim3(128), 0, st, J, K, n);
     HIPCHK(hipGetLastError());
+    if (b->timing) {
+        HIPCHK(hipEventRecord(b->ev_jk[1], st));
+        b->ev_jk_used = 1;
+    }
     return 0;
 }
 

@@ -9,6 +9,9 @@ coefficients from solving the bordered B system (diis.py:108-112).
 """
 import numpy as np
 
+# Works on numpy arrays or torch tensors: only @, *, sum and indexing are used
+# on the N x N matrices; the small B matrix is numpy.
+
 
 class DIIS:
     def __init__(self, threshold=0.5, max_condition=1e7, max_size=None):
@@ -25,10 +28,14 @@ class DIIS:
         """Xt (S D F - F D S) X, zero-padded to N x N (diis.py:48-54)."""
         r = S @ D @ F - F @ D @ S
         t = X.T @ r @ X
-        out = np.zeros_like(r)
+        out = r * 0.0
         n = t.shape[0]
         out[:n, :n] = t
         return out
+
+    @staticmethod
+    def _dot(a, b):
+        return float((a * b).sum())
 
     @staticmethod
     def _cond(m):
@@ -37,8 +44,10 @@ class DIIS:
 
     def extrapolate(self, res, F):
         self.error = float(res.max())
-        if abs(res.max()) >= abs(self.threshold):
+        if abs(self.error) >= abs(self.threshold):
             return F
+        if float(abs(res).max()) < 1e-14:
+            return F          # converged: nothing to extrapolate
         self.residuals.append(res)
         self.focks.append(F)
         if len(self.residuals) < 2:
@@ -49,10 +58,10 @@ class DIIS:
         if self.matrix is None:
             for i, r1 in enumerate(self.residuals):
                 for j, r2 in enumerate(self.residuals):
-                    B[i, j] = np.vdot(r1, r2)
+                    B[i, j] = self._dot(r1, r2)
         else:
             B[:size - 1, :size - 1] = self.matrix
-            new = [np.vdot(self.residuals[-1], r) for r in self.residuals]
+            new = [self._dot(self.residuals[-1], r) for r in self.residuals]
             B[:len(new), size - 2] = new
             B[size - 2, :len(new)] = new
         self.matrix = B
@@ -65,8 +74,11 @@ class DIIS:
             self.focks.pop(0)
         rhs = np.zeros(len(self.matrix))
         rhs[-1] = -1.0
-        coeffs = np.linalg.solve(self.matrix, rhs)[:-1]
-        out = np.zeros_like(F)
+        try:
+            coeffs = np.linalg.solve(self.matrix, rhs)[:-1]
+        except np.linalg.LinAlgError:
+            coeffs = np.linalg.lstsq(self.matrix, rhs, rcond=None)[0][:-1]
+        out = F * 0.0
         for c, f in zip(coeffs, self.focks):
-            out += c * f
+            out += float(c) * f
         return out

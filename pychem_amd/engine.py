@@ -86,13 +86,28 @@ class Engine:
         return out
 
     def build_eri(self, threshold=1e-8, fp32_threshold=0.0):
-        check(_lib.lib().pq_eri_build(self.h, float(threshold), float(fp32_threshold), _stream()), "pq_eri_build")
+        check(_lib.lib().pq_eri_plan(self.h, float(threshold), float(fp32_threshold), _stream()), "pq_eri_plan")
+        check(_lib.lib().pq_eri_compute(self.h, _stream()), "pq_eri_compute")
         self.eri_built = True
         self.threshold = threshold
         return self.stats()
 
     def recompute_eri(self):
         check(_lib.lib().pq_eri_recompute(self.h, _stream()), "pq_eri_recompute")
+
+    def set_timing(self, on=True):
+        check(_lib.lib().pq_set_timing(self.h, 1 if on else 0), "pq_set_timing")
+
+    def timings(self):
+        """(per-class ERI ms [nclass][2] of the last pass, ms of the last J/K)."""
+        nc = _lib.lib().pq_eri_nclass()
+        eri = np.zeros(2 * nc)
+        jk = np.zeros(1)
+        check(_lib.lib().pq_get_timings(self.h, dptr(eri), dptr(jk)), "pq_get_timings")
+        return eri.reshape(nc, 2), float(jk[0])
+
+    def run_class(self, cls, fp32=False):
+        check(_lib.lib().pq_eri_run_class(self.h, int(cls), 1 if fp32 else 0, _stream()), "pq_eri_run_class")
 
     def stats(self):
         nc = _lib.lib().pq_eri_nclass()

@@ -3,15 +3,19 @@
 
 Integrals, Schwarz screening, ERI storage and every J/K build run on the GPU
 (``Engine``); the FP64 Fock diagonalisation, DIIS and energy sums run on the
-host in numpy, as in the reference.  RHF is the UHF code path with equal
-alpha/beta occupations, exactly as the reference codes it
-(Methods/hartree_fock.py:23-154), so the same loop serves UHF.
+host, as in the reference, on torch CPU tensors with a bounded thread pool
+(one BLAS/LAPACK runtime, no oversubscription).  RHF is the UHF code path with
+equal alpha/beta occupations, exactly as the reference codes it
+(Methods/hartree_fock.py:23-154); when alpha and beta coincide the shared
+matrices are computed once.
 
 Functions keep the reference names: ``make_core_matrices``,
 ``evaluate_2e_ints``, ``make_coulomb_exchange_matrices``,
 ``make_fock_matrices``, ``make_MOs``, ``make_density_matrices``,
 ``calculate_energy``, ``do``.
 """
+import contextlib
+import os
 import time
 
 import numpy as np
@@ -21,22 +25,48 @@ from . import structures
 from .diis import DIIS
 from .engine import Engine
 
+# DIIS settings.  "reference": the reference's gate (a residual enters the
+# subspace only once max|r| < 0.5) and condition-number trimming at 1e7
+# (Methods/hf_extensions/diis.py:22,33,87-97).  With the core guess that gate
+# never opens for 6-31G* alkanes and the reference's loop oscillates forever;
+# "pulay" (default) is plain Pulay DIIS over the last 8 residuals.
+DIIS_MODES = {
+    "reference": dict(threshold=0.5, max_condition=1e7, max_size=None),
+    "pulay": dict(threshold=float("inf"), max_condition=1e14, max_size=8),
+}
+
+HOST = torch.device("cpu")
+F64 = torch.float64
+
+
+@contextlib.contextmanager
+def host_threads(n):
+    """Bound the host LAPACK/BLAS pool for N x N work (small matrices lose
+    time to large thread pools)."""
+    old = torch.get_num_threads()
+    want = int(os.environ.get("PYCHEM_HOST_THREADS", "0")) or (4 if n <= 256 else 8)
+    torch.set_num_threads(max(1, min(want, old)))
+    try:
+        yield
+    finally:
+        torch.set_num_threads(old)
+
 
 class Matrices:
     """Util/structures.py:882-901."""
 
     def __init__(self, n, nocc):
         self.NOcc = nocc
-        self.Density = np.zeros((n, n))
-        self.Fock = np.zeros((n, n))
-        self.Exchange = np.zeros((n, n))
+        self.Density = torch.zeros((n, n), dtype=F64)
+        self.Fock = torch.zeros((n, n), dtype=F64)
+        self.Exchange = torch.zeros((n, n), dtype=F64)
         self.MOs = None
         self.Energies = None
 
     def sort_occupied(self):
         """Sort the occupied orbitals by energy (Util/structures.py:895-901)."""
         n = self.NOcc
-        idx = np.append(np.argsort(self.Energies[:n]), np.arange(n, len(self.Energies)))
+        idx = torch.cat([torch.argsort(self.Energies[:n]), torch.arange(n, len(self.Energies))])
         self.Energies = self.Energies[idx]
         self.MOs = self.MOs[:, idx]
 
@@ -44,15 +74,15 @@ class Matrices:
 class ElectronicState:
     """Ground state with nalpha/nbeta electrons (Util/structures.py:862-876)."""
 
-    def __init__(self, n, nalpha, nbeta):
+    def __init__(self, n, nalpha, nbeta, diis_mode="pulay"):
         self.NAlpha, self.NBeta = nalpha, nbeta
         self.Alpha = Matrices(n, nalpha)
         self.Beta = Matrices(n, nbeta)
-        self.Coulomb = np.zeros((n, n))
+        self.Coulomb = torch.zeros((n, n), dtype=F64)
         self.Energy = None
         self.TotalEnergy = None
-        self.AlphaDIIS = DIIS()
-        self.BetaDIIS = DIIS()
+        self.AlphaDIIS = DIIS(**DIIS_MODES[diis_mode])
+        self.BetaDIIS = DIIS(**DIIS_MODES[diis_mode])
 
 
 def nuclear_repulsion(molecule):
@@ -79,14 +109,14 @@ def make_core_matrices(molecule):
     S^1/2 (Methods/hartree_fock.py:207-237)."""
     eng = get_engine(molecule)
     S, T, V = eng.one_electron()
-    molecule.Overlap = S.cpu().numpy()
-    molecule.Core = (T + V).cpu().numpy()
-    s, U = np.linalg.eigh(molecule.Overlap)
-    order = s.argsort()[::-1]
+    molecule.Overlap = S.to(HOST)
+    molecule.Core = (T + V).to(HOST)
+    s, U = torch.linalg.eigh(molecule.Overlap)
+    order = torch.argsort(s, descending=True)
     U, s = U[:, order], s[order]
     keep = s > structures.LINEAR_DEPENDENCE
-    molecule.X = U[:, keep] * (s[keep] ** -0.5)
-    molecule.Xt = molecule.X.T
+    molecule.X = (U[:, keep] * s[keep] ** -0.5).contiguous()
+    molecule.Xt = molecule.X.T.contiguous()
     molecule.S = (U[:, keep] * s[keep] ** 0.5) @ U[:, keep].T
 
 
@@ -98,85 +128,136 @@ def evaluate_2e_ints(molecule, threshold=structures.INTEGRAL_THRESHOLD, fp32_thr
     return eng.build_eri(threshold, fp32_threshold)
 
 
+def _record_jk_time(eng):
+    if getattr(eng, "timing_on", False):
+        eng.jk_ms.append(eng.timings()[1])
+
+
 def make_coulomb_exchange_matrices(molecule, state):
     """J from the total density, K^sigma = -sum D^sigma_cb (ab|cd)
     (Methods/hartree_fock.py:329-347)."""
     eng = get_engine(molecule)
     dev = eng.device
-    Da = torch.from_numpy(state.Alpha.Density).to(dev)
-    Dt = torch.from_numpy(state.Alpha.Density + state.Beta.Density).to(dev)
+    Dt = (state.Alpha.Density + state.Beta.Density).to(dev)
+    Da = state.Alpha.Density.to(dev)
     J, Ka = eng.jk(Dt, Da)
-    state.Coulomb = J.cpu().numpy()
-    state.Alpha.Exchange = Ka.cpu().numpy()
-    if state.NAlpha == state.NBeta and np.array_equal(state.Alpha.Density, state.Beta.Density):
+    _record_jk_time(eng)
+    state.Coulomb = J.to(HOST)
+    state.Alpha.Exchange = Ka.to(HOST)
+    if state.NAlpha == state.NBeta and (state.Alpha.Density is state.Beta.Density or
+                                        torch.equal(state.Alpha.Density, state.Beta.Density)):
         state.Beta.Exchange = state.Alpha.Exchange
     else:
-        _, Kb = eng.jk(Dt, torch.from_numpy(state.Beta.Density).to(dev))
-        state.Beta.Exchange = Kb.cpu().numpy()
+        _, Kb = eng.jk(Dt, state.Beta.Density.to(dev))
+        _record_jk_time(eng)
+        state.Beta.Exchange = Kb.to(HOST)
 
 
 def make_fock_matrices(molecule, state):
-    """Methods/hartree_fock.py:352-355."""
+    """Methods/hartree_fock.py:352-355 (one shared matrix when K^a is K^b)."""
     state.Alpha.Fock = molecule.Core + state.Coulomb + state.Alpha.Exchange
-    state.Beta.Fock = molecule.Core + state.Coulomb + state.Beta.Exchange
+    if state.Beta.Exchange is state.Alpha.Exchange:
+        state.Beta.Fock = state.Alpha.Fock
+    else:
+        state.Beta.Fock = molecule.Core + state.Coulomb + state.Beta.Exchange
+
+
+def _restricted(state):
+    return state.NAlpha == state.NBeta and state.Alpha.Fock is state.Beta.Fock
 
 
 def make_MOs(molecule, state):
-    """eigh of Xt F X (Methods/hartree_fock.py:168-175)."""
+    """eigh of Xt F X (Methods/hartree_fock.py:168-175); solved once when
+    alpha and beta share the Fock matrix."""
     X = molecule.X
-    for m in (state.Alpha, state.Beta):
-        e, C = np.linalg.eigh(molecule.Xt @ m.Fock @ X)
+    spins = (state.Alpha,) if _restricted(state) else (state.Alpha, state.Beta)
+    for m in spins:
+        e, C = torch.linalg.eigh(molecule.Xt @ m.Fock @ X)
         m.Energies, m.MOs = e, X @ C
+    if len(spins) == 1:
+        state.Beta.Energies, state.Beta.MOs = state.Alpha.Energies, state.Alpha.MOs
 
 
 def make_density_matrices(molecule, state):
     """D^sigma = C_occ C_occ^T (Methods/hartree_fock.py:179-184)."""
-    for m in (state.Alpha, state.Beta):
+    same = state.Alpha.MOs is state.Beta.MOs and state.NAlpha == state.NBeta
+    for m in ((state.Alpha,) if same else (state.Alpha, state.Beta)):
         C = m.MOs[:, :m.NOcc]
         m.Density = C @ C.T
+    if same:
+        state.Beta.Density = state.Alpha.Density
 
 
 def calculate_energy(molecule, state):
     """E = 1/2 [sum Dt H + sum Da Fa + sum Db Fb] (Methods/hartree_fock.py:188-201)."""
-    core = np.sum((state.Alpha.Density + state.Beta.Density) * molecule.Core)
-    state.Energy = 0.5 * (core + np.sum(state.Alpha.Density * state.Alpha.Fock)
-                          + np.sum(state.Beta.Density * state.Beta.Fock))
+    core = float(torch.sum((state.Alpha.Density + state.Beta.Density) * molecule.Core))
+    state.Energy = 0.5 * (core + float(torch.sum(state.Alpha.Density * state.Alpha.Fock))
+                          + float(torch.sum(state.Beta.Density * state.Beta.Fock)))
+
+
+def _as_host(M):
+    if isinstance(M, torch.Tensor):
+        return M.to(HOST, F64)
+    return torch.as_tensor(np.asarray(M, dtype=float))
 
 
 def do(molecule, conv=structures.ENERGY_CONVERGENCE, maxiter=25, use_diis=True, diis_start=1,
        guess="CORE", threshold=structures.INTEGRAL_THRESHOLD, fp32_threshold=0.0, initial_run=True,
-       verbose=False):
+       verbose=False, timing=False, diis_mode="pulay"):
     """SCF for the ground state (Methods/hartree_fock.py:23-154).
 
-    ``guess`` is "CORE" (the reference default), or an (N x N) density matrix
-    per spin given as a tuple (Da, Db), or a single density used for both.
-    Returns a dict with the energy, iteration count, convergence flag, MOs,
-    orbital energies, densities and per-stage timings.
+    ``guess`` is "CORE" (the reference default), "SAD", or an (N x N) density
+    matrix per spin given as a tuple (Da, Db), or a single density used for
+    both.  ``diis_mode`` is "pulay" (default) or "reference" (the reference's
+    gated DIIS).  Returns a dict with the energy, iteration count,
+    convergence flag, MOs, orbital energies, densities (numpy) and per-stage
+    timings.
     """
+    with host_threads(molecule.NOrbitals):
+        return _do(molecule, conv, maxiter, use_diis, diis_start, guess, threshold, fp32_threshold,
+                   initial_run, verbose, timing, diis_mode)
+
+
+def _do(molecule, conv, maxiter, use_diis, diis_start, guess, threshold, fp32_threshold, initial_run,
+        verbose, timing, diis_mode):
     t0 = time.perf_counter()
     timings = {}
+    if timing:
+        eng = get_engine(molecule)
+        eng.set_timing(True)
+        eng.timing_on = True
+        eng.jk_ms = []
     molecule.NuclearRepulsion = nuclear_repulsion(molecule)
     make_core_matrices(molecule)
     timings["one_electron"] = time.perf_counter() - t0
     n = molecule.NOrbitals
-    state = ElectronicState(n, molecule.NAlphaElectrons, molecule.NBetaElectrons)
-    state.Alpha.Fock = molecule.Core.copy()
-    state.Beta.Fock = molecule.Core.copy()
+    state = ElectronicState(n, molecule.NAlphaElectrons, molecule.NBetaElectrons, diis_mode)
+    state.Alpha.Fock = molecule.Core.clone()
+    state.Beta.Fock = state.Alpha.Fock if state.NAlpha == state.NBeta else molecule.Core.clone()
     t1 = time.perf_counter()
     if initial_run:
         evaluate_2e_ints(molecule, threshold, fp32_threshold)
         torch.cuda.synchronize()
     timings["two_electron"] = time.perf_counter() - t1
+    if timing:
+        timings["eri_class_ms"] = get_engine(molecule).timings()[0]
 
-    if isinstance(guess, str):
+    if isinstance(guess, str) and guess == "SAD":
+        from .guess import sad_density
+        t3 = time.perf_counter()
+        Dsad = _as_host(sad_density(molecule))
+        timings["guess"] = time.perf_counter() - t3
+        state.Alpha.Density = 0.5 * Dsad
+        state.Beta.Density = state.Alpha.Density
+    elif isinstance(guess, str):
         if guess != "CORE":
             raise ValueError("unknown guess %r" % guess)
         make_MOs(molecule, state)
         make_density_matrices(molecule, state)
     else:
         Da, Db = guess if isinstance(guess, tuple) else (guess, guess)
-        state.Alpha.Density = np.array(Da, dtype=float)
-        state.Beta.Density = np.array(Db, dtype=float)
+        state.Alpha.Density = _as_host(Da)
+        state.Beta.Density = state.Alpha.Density if Db is Da else _as_host(Db)
     calculate_energy(molecule, state)
     dE = 1.0
     it = 0
@@ -190,13 +271,23 @@ def do(molecule, conv=structures.ENERGY_CONVERGENCE, maxiter=25, use_diis=True, 
         make_coulomb_exchange_matrices(molecule, state)
         make_fock_matrices(molecule, state)
         if use_diis and it > diis_start:
-            for m, d in ((state.Alpha, state.AlphaDIIS), (state.Beta, state.BetaDIIS)):
-                r = DIIS.residual(molecule.Overlap, m.Density, m.Fock, molecule.X)
-                m.Fock = d.extrapolate(r, m.Fock)
-            diis_error = max(state.AlphaDIIS.error, state.BetaDIIS.error)
+            if _restricted(state):
+                r = DIIS.residual(molecule.Overlap, state.Alpha.Density, state.Alpha.Fock, molecule.X)
+                state.Alpha.Fock = state.AlphaDIIS.extrapolate(r, state.Alpha.Fock)
+                state.Beta.Fock = state.Alpha.Fock
+                diis_error = state.AlphaDIIS.error
+            else:
+                for m, d in ((state.Alpha, state.AlphaDIIS), (state.Beta, state.BetaDIIS)):
+                    r = DIIS.residual(molecule.Overlap, m.Density, m.Fock, molecule.X)
+                    m.Fock = d.extrapolate(r, m.Fock)
+                diis_error = max(state.AlphaDIIS.error, state.BetaDIIS.error)
         make_MOs(molecule, state)
+        shared = state.Beta.MOs is state.Alpha.MOs
         state.Alpha.sort_occupied()
-        state.Beta.sort_occupied()
+        if shared:
+            state.Beta.MOs, state.Beta.Energies = state.Alpha.MOs, state.Alpha.Energies
+        else:
+            state.Beta.sort_occupied()
         make_density_matrices(molecule, state)
         old = state.Energy
         calculate_energy(molecule, state)
@@ -206,11 +297,17 @@ def do(molecule, conv=structures.ENERGY_CONVERGENCE, maxiter=25, use_diis=True, 
         if verbose:
             print(" Cycle: %d  Total energy: %.12f  Change: %.3e  DIIS error: %s" % (it, state.TotalEnergy, dE, diis_error))
     timings["scf_loop"] = time.perf_counter() - t2
+    if timing:
+        timings["jk_ms"] = list(get_engine(molecule).jk_ms)
     timings["total"] = time.perf_counter() - t0
     if state.TotalEnergy is None:
         state.TotalEnergy = state.Energy + molecule.NuclearRepulsion
     molecule.State = state
+
+    def npy(x):
+        return None if x is None else x.numpy()
     return dict(energy=state.TotalEnergy, electronic=state.Energy, iterations=it,
-                converged=abs(dE) <= conv, Ca=state.Alpha.MOs, Cb=state.Beta.MOs,
-                ea=state.Alpha.Energies, eb=state.Beta.Energies, Da=state.Alpha.Density,
-                Db=state.Beta.Density, history=history, timings=timings, nuclear=molecule.NuclearRepulsion)
+                converged=abs(dE) <= conv, Ca=npy(state.Alpha.MOs), Cb=npy(state.Beta.MOs),
+                ea=npy(state.Alpha.Energies), eb=npy(state.Beta.Energies), Da=npy(state.Alpha.Density),
+                Db=npy(state.Beta.Density), history=history, timings=timings,
+                nuclear=molecule.NuclearRepulsion)

@@ -161,7 +161,7 @@ class Molecule:
     """
 
     def __init__(self, coords, basis="STO3G", charge=0, multiplicity=1, cartesian_l=(),
-                 units="ANGSTROM", build_pairs=True):
+                 units="ANGSTROM", build_pairs=False):
         self.Coords = [list(r) for r in coords]
         self.NAtom = len(coords)
         self.Charge = int(charge)
@@ -188,7 +188,7 @@ class Molecule:
                 self.Shells.append(Shell(atom.Coordinates, cg, len(self.Shells),
                                          list(range(off, off + cg.NAngMom))))
                 off += cg.NAngMom
-        self.ShellPairs = None
+        self._shell_pairs = None
         if build_pairs:
             self.build_shell_pairs()
         self.Core = None
@@ -202,8 +202,16 @@ class Molecule:
             for b in range(a, len(self.Shells)):
                 sb = self.Shells[b]
                 sp[(a, b)] = ShellPair(sa.Coords, sa.Cgtf, a, sa.Ivec, sb.Coords, sb.Cgtf, b, sb.Ivec)
-        self.ShellPairs = sp
+        self._shell_pairs = sp
         return sp
+
+    @property
+    def ShellPairs(self):
+        """Python ShellPair objects (built on first use; the HIP engine builds
+        its own device-side pair data)."""
+        if self._shell_pairs is None:
+            self.build_shell_pairs()
+        return self._shell_pairs
 
     # ------------------------------------------------------------------ #
     def shell_table(self):

