@@ -107,6 +107,35 @@ int pq_eri_dense(pq_basis* b, double* out_dev, void* stream);
  * (Methods/integrals.py:427-555) before its spherical transform. */
 int pq_eri_blocks(pq_basis* b, int n, const int* shells, double* out_host, void* stream);
 
+/* Slab of the dense ERI tensor: out[mu][nu][lam - l0][sig] for lam in
+ * [l0, l0 + lc) (zero where screened).  lc = ncart gives the full tensor. */
+int pq_eri_dense_slab(pq_basis* b, int l0, int lc, double* out_dev, void* stream);
+
+/* Batched FP64 matrix-core GEMM: C[b] = alpha op(A[b]) op(B[b]) + beta C[b],
+ * op(A)(m,k) = ta ? A[k*lda+m] : A[m*lda+k], op(B)(k,n) = tb ? B[n*ldb+k] :
+ * B[k*ldb+n]; batch strides sA, sB, sC in elements.  The dense products of
+ * the MP2 transform (Methods/mp2.py:43-69 uses numpy dot). */
+int pq_gemm_f64(int ta, int tb, int M, int N, int K, double alpha, const double* A, long long lda,
+                long long sA, const double* B, long long ldb, long long sB, double beta, double* C,
+                long long ldc, long long sC, int batch, void* stream);
+
+/* AO->MO transform (ia|jb) = sum C1_mi C1v_na C2_lj C2v_sb (mn|ls) from the
+ * stored ERIs, by quarter transforms over slabs of lam (at most
+ * max_slab_bytes of dense AO integrals at a time; <= 0 means 4 GiB).
+ * C matrices are row-major over Cartesian AOs: Co1[ncart][o1] etc.
+ * ovov_dev is [o1][v1][o2][v2].  Replaces the O(N^6) loops of
+ * Methods/mp2.py:37-69 (one call per spin pair: aa, bb, ab). */
+int pq_mp2_ovov(pq_basis* b, int o1, int v1, const double* Co1, const double* Cv1, int o2, int v2,
+                const double* Co2, const double* Cv2, double* ovov_dev, long long max_slab_bytes,
+                void* stream);
+
+/* MP2 energy sum over ovov [o1][v1][o2][v2] with D = e_i + e_j - e_a - e_b:
+ * mode 0 closed shell sum v(2v - (ib|ja))/D; mode 1 same spin
+ * 1/4 sum ((ia|jb) - (ib|ja))^2 / D; mode 2 opposite spin sum v^2 / D
+ * (Methods/mp2.py:77-94).  Orbital energies are device arrays. */
+int pq_mp2_energy(const double* ovov_dev, const double* eo1, const double* ev1, const double* eo2,
+                  const double* ev2, int o1, int v1, int o2, int v2, int mode, double* out_host, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

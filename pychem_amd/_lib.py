@@ -40,6 +40,14 @@ SIGNATURES = {
     "pq_jk": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pq_eri_dense": (c_int, [c_void_p, c_void_p, c_void_p]),
     "pq_eri_blocks": (c_int, [c_void_p, c_int, P_int, P_double, c_void_p]),
+    "pq_eri_dense_slab": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "pq_gemm_f64": (c_int, [c_int, c_int, c_int, c_int, c_int, c_double, c_void_p, c_longlong, c_longlong,
+                            c_void_p, c_longlong, c_longlong, c_double, c_void_p, c_longlong, c_longlong,
+                            c_int, c_void_p]),
+    "pq_mp2_ovov": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
+                            c_void_p, c_longlong, c_void_p]),
+    "pq_mp2_energy": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
+                              c_int, P_double, c_void_p]),
 }
 
 

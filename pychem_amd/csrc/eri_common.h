@@ -59,8 +59,8 @@ struct ScatterArgs {
     const PairHdr* pairs;
     const int* cart_off;
     const real* vals;
-    double* out;        // n^4 dense
-    int nq, n;
+    double* out;        // out[m][n][l - l0][s] for l in [l0, l0 + lc)
+    int nq, n, l0, lc;
 };
 
 // --------------------------------------------------------------------------

@@ -232,7 +232,8 @@ __global__ void __launch_bounds__(64) jk_run_kernel(const JkRunArgs<real> a)
     }
 }
 
-// Dense scatter with 8-fold permutational symmetry (Methods/hartree_fock.py:314-325).
+// Dense scatter with 8-fold permutational symmetry (Methods/hartree_fock.py:314-325)
+// into a slab of third indices l in [l0, l0 + lc): out[m][n][l - l0][s].
 template <int LA, int LB, int LC, int LD, typename real>
 __global__ void __launch_bounds__(128) scatter_kernel(const ScatterArgs<real> a)
 {
@@ -242,22 +243,26 @@ __global__ void __launch_bounds__(128) scatter_kernel(const ScatterArgs<real> a)
     if (q >= a.nq) return;
     const int2 pr = a.quartets[q];
     const PairHdr bh = a.pairs[pr.x], kh = a.pairs[pr.y];
-    const size_t o1 = a.cart_off[bh.s1], o2 = a.cart_off[bh.s2];
-    const size_t o3 = a.cart_off[kh.s1], o4 = a.cart_off[kh.s2];
-    const size_t n = a.n, nq = a.nq;
+    const long long o1 = a.cart_off[bh.s1], o2 = a.cart_off[bh.s2];
+    const long long o3 = a.cart_off[kh.s1], o4 = a.cart_off[kh.s2];
+    const long long n = a.n, nq = a.nq, l0 = a.l0, lc = a.lc;
+    auto put = [&](long long m, long long nn, long long l, long long s, double v) {
+        const long long t = l - l0;
+        if (t >= 0 && t < lc) a.out[((m * n + nn) * lc + t) * n + s] = v;
+    };
     for (int ia = 0; ia < N1; ++ia)
         for (int ib = 0; ib < N2; ++ib)
             for (int ic = 0; ic < N3; ++ic)
                 for (int id = 0; id < N4; ++id) {
                     const double v = (double)a.vals[(size_t)(((ia * N2 + ib) * N3 + ic) * N4 + id) * nq + q];
-                    const size_t m = o1 + ia, nn = o2 + ib, l = o3 + ic, s = o4 + id;
-                    a.out[((m * n + nn) * n + l) * n + s] = v;
-                    a.out[((nn * n + m) * n + l) * n + s] = v;
-                    a.out[((m * n + nn) * n + s) * n + l] = v;
-                    a.out[((nn * n + m) * n + s) * n + l] = v;
-                    a.out[((l * n + s) * n + m) * n + nn] = v;
-                    a.out[((l * n + s) * n + nn) * n + m] = v;
-                    a.out[((s * n + l) * n + m) * n + nn] = v;
-                    a.out[((s * n + l) * n + nn) * n + m] = v;
+                    const long long m = o1 + ia, nn = o2 + ib, l = o3 + ic, s = o4 + id;
+                    put(m, nn, l, s, v);
+                    put(nn, m, l, s, v);
+                    put(m, nn, s, l, v);
+                    put(nn, m, s, l, v);
+                    put(l, s, m, nn, v);
+                    put(l, s, nn, m, v);
+                    put(s, l, m, nn, v);
+                    put(s, l, nn, m, v);
                 }
 }

@@ -23,6 +23,7 @@
 #include "../../include/pychem_hip.h"
 #include "eri_common.h"
 #include "eri_dispatch.h"
+#include "mp2.h"
 
 
 __global__ void boys_test_kernel(const double* T, int n, int mmax, double* out)
@@ -709,11 +710,11 @@ extern "C" int pq_eri_dense(pq_basis* b, double* out, void* stream)
             if (!s.nq) continue;
             if (w) {
                 ScatterArgs<float> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p,
-                                     (const float*)s.vals.p, out, s.nq, (int)n};
+                                     (const float*)s.vals.p, out, s.nq, (int)n, 0, (int)n};
                 scatter_launch<float>(c, a, st);
             } else {
                 ScatterArgs<double> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p,
-                                      (const double*)s.vals.p, out, s.nq, (int)n};
+                                      (const double*)s.vals.p, out, s.nq, (int)n, 0, (int)n};
                 scatter_launch<double>(c, a, st);
             }
         }
@@ -799,5 +800,114 @@ extern "C" int pq_eri_blocks(pq_basis* b, int n, const int* shells, double* out_
                         }
         }
     }
+    return 0;
+}
+
+// --------------------------------------------------------------------------
+// MP2 AO->MO transform
+// --------------------------------------------------------------------------
+static int eri_dense_slab(pq_basis* b, int l0, int lc, double* out, hipStream_t st)
+{
+    const size_t n = b->ncart;
+    HIPCHK(hipMemsetAsync(out, 0, sizeof(double) * n * n * (size_t)lc * n, st));
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        for (int w = 0; w < 2; ++w) {
+            EriClassStore& s = w ? b->store32[c] : b->store64[c];
+            if (!s.nq) continue;
+            if (w) {
+                ScatterArgs<float> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p,
+                                     (const float*)s.vals.p, out, s.nq, (int)n, l0, lc};
+                scatter_launch<float>(c, a, st);
+            } else {
+                ScatterArgs<double> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p,
+                                      (const double*)s.vals.p, out, s.nq, (int)n, l0, lc};
+                scatter_launch<double>(c, a, st);
+            }
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int pq_eri_dense_slab(pq_basis* b, int l0, int lc, double* out, void* stream)
+{
+    if (!b || !out) return fail("pq_eri_dense_slab: null argument");
+    if (l0 < 0 || lc <= 0 || l0 + lc > b->ncart) return fail("pq_eri_dense_slab: slab out of range");
+    return eri_dense_slab(b, l0, lc, out, (hipStream_t)stream);
+}
+
+extern "C" int pq_gemm_f64(int ta, int tb, int M, int N, int K, double alpha, const double* A, long long lda,
+                           long long sA, const double* B, long long ldb, long long sB, double beta, double* C,
+                           long long ldc, long long sC, int batch, void* stream)
+{
+    if (!A || !B || !C) return fail("pq_gemm_f64: null argument");
+    if (M < 0 || N < 0 || K <= 0 || batch < 0) return fail("pq_gemm_f64: bad sizes");
+    GemmF64Args g{A, B, C, lda, ldb, ldc, sA, sB, sC, M, N, K, ta, tb, alpha, beta};
+    if (gemm_f64_launch(g, batch, (hipStream_t)stream)) return fail("pq_gemm_f64: launch failed");
+    return 0;
+}
+
+extern "C" int pq_mp2_ovov(pq_basis* b, int o1, int v1, const double* Co1, const double* Cv1, int o2, int v2,
+                           const double* Co2, const double* Cv2, double* ovov, long long max_slab_bytes,
+                           void* stream)
+{
+    if (!b || !Co1 || !Cv1 || !Co2 || !Cv2 || !ovov) return fail("pq_mp2_ovov: null argument");
+    if (o1 <= 0 || v1 <= 0 || o2 <= 0 || v2 <= 0) return fail("pq_mp2_ovov: empty orbital space");
+    hipStream_t st = (hipStream_t)stream;
+    const long long n = b->ncart;
+    if (max_slab_bytes <= 0) max_slab_bytes = 4LL << 30;
+    long long lc = max_slab_bytes / (8 * n * n * n);
+    lc = lc < 1 ? 1 : (lc > n ? n : lc);
+    DevBuf slab, t1, t2, t3;
+    if (slab.ensure(sizeof(double) * n * n * lc * n) || t1.ensure(sizeof(double) * o1 * n * lc * n) ||
+        t2.ensure(sizeof(double) * (size_t)o1 * v1 * n * n) || t3.ensure(sizeof(double) * (size_t)o1 * v1 * o2 * n)) {
+        slab.release(); t1.release(); t2.release(); t3.release();
+        return -1;
+    }
+    int rc = 0;
+    for (long long l0 = 0; l0 < n && !rc; l0 += lc) {
+        const long long L = (l0 + lc <= n) ? lc : n - l0;
+        const long long X = n * L * n, Y = L * n;
+        rc |= eri_dense_slab(b, (int)l0, (int)L, (double*)slab.p, st);
+        // (i nu | lam sig) = sum_mu C1[mu][i] (mu nu | lam sig)
+        GemmF64Args ga{Co1, (const double*)slab.p, (double*)t1.p, o1, X, X, 0, 0, 0, o1, (int)X, (int)n, 1, 0, 1.0, 0.0};
+        rc |= gemm_f64_launch(ga, 1, st);
+        // (i a | lam sig) = sum_nu C1v[nu][a] (i nu | lam sig), written into the full [i][a][n][n] array
+        GemmF64Args gb{Cv1, (const double*)t1.p, (double*)t2.p + l0 * n, v1, Y, n * n, 0, n * Y, (long long)v1 * n * n,
+                       v1, (int)Y, (int)n, 1, 0, 1.0, 0.0};
+        rc |= gemm_f64_launch(gb, o1, st);
+    }
+    if (!rc) {
+        // (i a | j sig) = sum_lam C2[lam][j] (i a | lam sig)
+        GemmF64Args gc{Co2, (const double*)t2.p, (double*)t3.p, o2, n, n, 0, n * n, (long long)o2 * n, o2, (int)n,
+                       (int)n, 1, 0, 1.0, 0.0};
+        rc |= gemm_f64_launch(gc, o1 * v1, st);
+        // (i a | j b) = sum_sig (i a | j sig) C2v[sig][b]
+        GemmF64Args gd{(const double*)t3.p, Cv2, ovov, n, v2, v2, 0, 0, 0, o1 * v1 * o2, v2, (int)n, 0, 0, 1.0, 0.0};
+        rc |= gemm_f64_launch(gd, 1, st);
+    }
+    hipError_t e = hipStreamSynchronize(st);
+    slab.release(); t1.release(); t2.release(); t3.release();
+    if (rc) return fail("pq_mp2_ovov: kernel launch failed");
+    if (e != hipSuccess) return fail(std::string("pq_mp2_ovov: ") + hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int pq_mp2_energy(const double* ovov, const double* eo1, const double* ev1, const double* eo2,
+                             const double* ev2, int o1, int v1, int o2, int v2, int mode, double* out_host,
+                             void* stream)
+{
+    if (!ovov || !eo1 || !ev1 || !eo2 || !ev2 || !out_host) return fail("pq_mp2_energy: null argument");
+    if (mode < 0 || mode > 2) return fail("pq_mp2_energy: mode must be 0, 1 or 2");
+    if (mode != 2 && (o1 != o2 || v1 != v2)) return fail("pq_mp2_energy: modes 0/1 need equal spaces");
+    hipStream_t st = (hipStream_t)stream;
+    DevBuf acc;
+    if (acc.ensure(sizeof(double))) return -1;
+    HIPCHK(hipMemsetAsync(acc.p, 0, sizeof(double), st));
+    if (mp2_energy_launch(ovov, eo1, ev1, eo2, ev2, o1, v1, o2, v2, mode, (double*)acc.p, st))
+        return fail("pq_mp2_energy: launch failed");
+    HIPCHK(hipMemcpyAsync(out_host, acc.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    acc.release();
     return 0;
 }

@@ -157,6 +157,52 @@ class Engine:
         return res
 
 
+    def eri_dense_slab(self, l0, lc):
+        n = self.ncart
+        out = torch.empty((n, n, lc, n), dtype=torch.float64, device=self.device)
+        check(_lib.lib().pq_eri_dense_slab(self.h, int(l0), int(lc), _ptr(out), _stream()), "pq_eri_dense_slab")
+        return out
+
+    def mp2_ovov(self, Co1, Cv1, Co2, Cv2, max_slab_bytes=0):
+        """(ia|jb) [o1][v1][o2][v2] from MO coefficient blocks given over the
+        final basis (rows = AOs); Cartesian rows are formed here."""
+        mats = []
+        for C in (Co1, Cv1, Co2, Cv2):
+            C = torch.as_tensor(C, dtype=torch.float64, device=self.device)
+            if not self.cartesian:
+                C = self.T.T @ C
+            mats.append(C.contiguous())
+        o1, v1, o2, v2 = mats[0].shape[1], mats[1].shape[1], mats[2].shape[1], mats[3].shape[1]
+        out = torch.empty((o1, v1, o2, v2), dtype=torch.float64, device=self.device)
+        check(_lib.lib().pq_mp2_ovov(self.h, o1, v1, _ptr(mats[0]), _ptr(mats[1]), o2, v2, _ptr(mats[2]),
+                                     _ptr(mats[3]), _ptr(out), int(max_slab_bytes), _stream()), "pq_mp2_ovov")
+        return out
+
+
+def mp2_energy(ovov, eo1, ev1, eo2, ev2, mode):
+    """MP2 energy reduction (mode 0 closed shell, 1 same spin, 2 opposite spin)."""
+    dev = ovov.device
+    es = [torch.as_tensor(np.asarray(e, dtype=float), device=dev).contiguous() for e in (eo1, ev1, eo2, ev2)]
+    o1, v1, o2, v2 = ovov.shape
+    out = np.zeros(1)
+    check(_lib.lib().pq_mp2_energy(_ptr(ovov), _ptr(es[0]), _ptr(es[1]), _ptr(es[2]), _ptr(es[3]),
+                                   o1, v1, o2, v2, int(mode), dptr(out), _stream()), "pq_mp2_energy")
+    return float(out[0])
+
+
+def gemm(A, B, ta=False, tb=False, alpha=1.0):
+    """C = op(A) op(B) on the matrix cores (FP64), torch device tensors."""
+    A = A.contiguous()
+    B = B.contiguous()
+    M = A.shape[1] if ta else A.shape[0]
+    K = A.shape[0] if ta else A.shape[1]
+    N = B.shape[0] if tb else B.shape[1]
+    C = torch.empty((M, N), dtype=torch.float64, device=A.device)
+    check(_lib.lib().pq_gemm_f64(int(ta), int(tb), M, N, K, float(alpha), _ptr(A), A.shape[1], 0, _ptr(B),
+                                 B.shape[1], 0, 0.0, _ptr(C), N, 0, 1, _stream()), "pq_gemm_f64")
+    return C
+
+
 def boys(T, mmax):
     """Device Boys function (test entry point): returns numpy [n][mmax+1]."""
     Tt = torch.as_tensor(np.asarray(T, dtype=float).reshape(-1), device="cuda")

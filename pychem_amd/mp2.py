@@ -1,0 +1,67 @@
+"""MP2 on the HIP engine, with the reference's call surface (Methods/mp2.py).
+
+The AO->MO transform is the O(N^5) quarter transform on the matrix cores
+(pq_mp2_ovov) over the stored screened ERIs; the reference's O(N^6) Python
+loops (Methods/mp2.py:43-69) compute the same (ia|jb).  Energies:
+
+* RHF (alpha == beta): E = sum (ia|jb)[2(ia|jb) - (ib|ja)] / D, identical to
+  the reference's spin-orbital sums for a closed shell.
+* UHF: same-spin 1/4 sum [(ia|jb) - (ib|ja)]^2 / D for alpha and beta
+  (= the reference's i>=j, p>=q sums, Methods/mp2.py:77-88) plus
+  opposite-spin sum (ia|jb)^2 / D (Methods/mp2.py:90-94).
+
+Spin-component scaling uses c_ss = 1/3, c_os = 6/5 (SCS) and c_os = 1.3 (SOS).
+(The reference writes 1/3 and 6/5 under Python 2 integer division, which
+evaluates to 0 and 1; the intended factors are used here.)
+"""
+import numpy as np
+import torch
+
+from .engine import mp2_energy
+
+
+def do(molecule, result=None, method="MP2", max_slab_bytes=0):
+    """Correlation energy for the converged SCF ``result`` of
+    ``hartree_fock.do`` (uses molecule.Engine and its stored ERIs).  Returns
+    a dict with the correlation energy, the total energy and the parts."""
+    eng = molecule.Engine
+    st = molecule.State if result is None else None
+    Ca = result["Ca"] if result else st.Alpha.MOs.numpy()
+    Cb = result["Cb"] if result else st.Beta.MOs.numpy()
+    ea = result["ea"] if result else st.Alpha.Energies.numpy()
+    eb = result["eb"] if result else st.Beta.Energies.numpy()
+    etot = result["energy"] if result else st.TotalEnergy
+    na, nb = molecule.NAlphaElectrons, molecule.NBetaElectrons
+    restricted = na == nb and np.array_equal(Ca, Cb)
+    parts = {}
+    if restricted:
+        ov = eng.mp2_ovov(Ca[:, :na], Ca[:, na:], Ca[:, :na], Ca[:, na:], max_slab_bytes)
+        e_cs = mp2_energy(ov, ea[:na], ea[na:], ea[:na], ea[na:], 0)
+        e_os = mp2_energy(ov, ea[:na], ea[na:], ea[:na], ea[na:], 2)
+        e_ss_half = mp2_energy(ov, ea[:na], ea[na:], ea[:na], ea[na:], 1)
+        parts = dict(aa=e_ss_half, bb=e_ss_half, ab=e_os)
+        del ov
+        e_plain = e_cs
+    else:
+        ov = eng.mp2_ovov(Ca[:, :na], Ca[:, na:], Ca[:, :na], Ca[:, na:], max_slab_bytes)
+        parts["aa"] = mp2_energy(ov, ea[:na], ea[na:], ea[:na], ea[na:], 1)
+        del ov
+        if nb > 0:
+            ov = eng.mp2_ovov(Cb[:, :nb], Cb[:, nb:], Cb[:, :nb], Cb[:, nb:], max_slab_bytes)
+            parts["bb"] = mp2_energy(ov, eb[:nb], eb[nb:], eb[:nb], eb[nb:], 1)
+            del ov
+            ov = eng.mp2_ovov(Ca[:, :na], Ca[:, na:], Cb[:, :nb], Cb[:, nb:], max_slab_bytes)
+            parts["ab"] = mp2_energy(ov, ea[:na], ea[na:], eb[:nb], eb[nb:], 2)
+            del ov
+        else:
+            parts["bb"] = parts["ab"] = 0.0
+        e_plain = parts["aa"] + parts["bb"] + parts["ab"]
+    m = method.upper()
+    if "SCS" in m:
+        ecorr = (parts["aa"] + parts["bb"]) / 3.0 + 1.2 * parts["ab"]
+    elif "SOS" in m:
+        ecorr = 1.3 * parts["ab"]
+    else:
+        ecorr = e_plain
+    torch.cuda.synchronize()
+    return dict(correlation=ecorr, total=etot + ecorr, parts=parts)
