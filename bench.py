@@ -46,14 +46,42 @@ def flop_model():
         return json.load(fh)
 
 
-def gpu_solve(mol_factory, timing=True, guess_kind="SAD"):
+def gpu_solve(mol_factory, timing=True, guess_kind="SAD", fp32_threshold=0.0, keep=False):
     from pychem_amd import guess, hartree_fock
     guess._ATOM_CACHE.clear()          # atomic SCFs are part of every solve
     mol = mol_factory()
-    r = hartree_fock.do(mol, conv=1e-10, maxiter=100, timing=timing, guess=guess_kind)
+    r = hartree_fock.do(mol, conv=1e-10, maxiter=100, timing=timing, guess=guess_kind,
+                        fp32_threshold=fp32_threshold)
     stats = mol.Engine.stats()
+    if keep:
+        return r, stats, mol
     mol.Engine.close()
     return r, stats
+
+
+def mp2_measure(mol, r):
+    """Time the MP2 transform + energy on the converged RHF (device events)."""
+    import torch
+    from pychem_amd import mp2
+    from pychem_amd.engine import mp2_energy
+    eng = mol.Engine
+    no = mol.NAlphaElectrons
+    C, e = r["Ca"], r["ea"]
+    n = C.shape[0]
+    v = n - no
+    mp2.do(mol, r)                      # warm-up (allocations, code objects)
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    ov = eng.mp2_ovov(C[:, :no], C[:, no:], C[:, :no], C[:, no:])
+    t1.record()
+    ecorr = mp2_energy(ov, e[:no], e[no:], e[:no], e[no:], 0)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1)
+    flops = 2.0 * no * n ** 4 + 2.0 * no * v * n ** 3 + 2.0 * no * no * v * n ** 2 + 2.0 * no * no * v * v * n
+    return {"mp2_correlation": ecorr, "transform_ms": ms, "flops": flops,
+            "tflops": flops / (ms * 1e-3) / 1e12, "frac_fp64_peak": flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+            "nocc": no, "nvir": v}
 
 
 def cpu_baseline(name, threads):
@@ -104,6 +132,17 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
 
     r0, stats = results[-1]
+    # MP2 on the same molecule (outside the timed RHF steps)
+    rk, _, molk = gpu_solve(factory, keep=True)
+    mp2_info = mp2_measure(molk, rk)
+    molk.Engine.close()
+    # mixed precision: Schwarz-split FP32 ERIs below a bound of 1e-6
+    mixed = []
+    t0m = time.perf_counter()
+    for _ in range(args.steps):
+        mixed.append(gpu_solve(factory, fp32_threshold=1e-6))
+    torch.cuda.synchronize()
+    mixed_s = (time.perf_counter() - t0m) / args.steps
     nq = sum(s["nq64"] + s["nq32"] for s in stats)
     nprim = sum(s["nprim64"] + s["nprim32"] for s in stats)
     value = nq * args.steps / wall
@@ -144,6 +183,14 @@ def main():
         "stage_s": {k: v for k, v in r0["timings"].items() if isinstance(v, float)},
         "roofline": roofline,
         "per_class": per_class,
+        "mp2": mp2_info,
+        "mixed": {"time_to_solution_s": mixed_s, "speedup_vs_fp64": (ms_per_step / 1e3) / mixed_s,
+                  "energy_error_vs_fp64": mixed[-1][0]["energy"] - r0["energy"],
+                  "scf_iterations": mixed[-1][0]["iterations"],
+                  "fp32_quartets": sum(s["nq32"] for s in mixed[-1][1]),
+                  "fp64_quartets": sum(s["nq64"] for s in mixed[-1][1]),
+                  "eri_pass_ms": float(np.sum(np.clip(mixed[-1][0]["timings"]["eri_class_ms"], 0, None))),
+                  "jk_ms": float(np.mean(mixed[-1][0]["timings"]["jk_ms"]))},
     }
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))

@@ -25,7 +25,7 @@ template <typename real>
 struct EriArgs {
     const int2* quartets;
     const PairHdr* pairs;
-    const PrimPair<real>* prims;
+    const PrimPair<double>* prims;   // FP64 for both precisions
     real* out;
     int nq;
 };
@@ -132,21 +132,26 @@ __device__ __forceinline__ void boys_eval(float T, float* F)
 
 // Per primitive quartet: prefactor and recurrence coefficients for the
 // unscaled Obara-Saika VRR.  W - P = -(q/(p+q)) (P-Q), W - Q = (p/(p+q)) (P-Q).
+// The geometry (P-Q, T, prefactor) is always formed in FP64 from the FP64
+// primitive-pair data; the FP32 kernels round only after that, so centre
+// coordinates never lose digits to single precision.
 #define ERI_PRIM_SETUP(LL)                                                          \
-    const real p_ = P.p, q_ = Q.p;                                                  \
-    const real ipq = (real)1 / (p_ + q_);                                           \
-    const real rho = p_ * q_ * ipq;                                                 \
-    const real PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;             \
-    const real Tb = rho * (PQx * PQx + PQy * PQy + PQz * PQz);                      \
+    const double pd_ = P.p, qd_ = Q.p;                                              \
+    const double ipq_d = 1.0 / (pd_ + qd_);                                         \
+    const double rho_d = pd_ * qd_ * ipq_d;                                         \
+    const double PQx_d = P.Px - Q.Px, PQy_d = P.Py - Q.Py, PQz_d = P.Pz - Q.Pz;     \
+    const double T_d = rho_d * (PQx_d * PQx_d + PQy_d * PQy_d + PQz_d * PQz_d);     \
     real F[(LL) + 1];                                                               \
-    boys_eval<(LL)>(Tb, F);                                                         \
-    const real pref = P.K * Q.K * (real)1.1283791670955126 * sqrt(rho);             \
-    const real rho_p = q_ * ipq, rho_q = p_ * ipq;                                  \
+    boys_eval<(LL)>((real)T_d, F);                                                  \
+    const real pref = (real)(P.K * Q.K * 1.1283791670955126 * sqrt(rho_d));         \
+    const real ipq = (real)ipq_d;                                                   \
+    const real rho_p = (real)(qd_ * ipq_d), rho_q = (real)(pd_ * ipq_d);            \
+    const real PQx = (real)PQx_d, PQy = (real)PQy_d, PQz = (real)PQz_d;             \
     const real WPx = -rho_p * PQx, WPy = -rho_p * PQy, WPz = -rho_p * PQz;          \
     const real WQx = rho_q * PQx, WQy = rho_q * PQy, WQz = rho_q * PQz;             \
-    const real PAx = P.PAx, PAy = P.PAy, PAz = P.PAz;                               \
-    const real QCx = Q.PAx, QCy = Q.PAy, QCz = Q.PAz;                               \
-    const real oo2p = P.inv2p, oo2q = Q.inv2p, oo2pq = (real)0.5 * ipq;             \
+    const real PAx = (real)P.PAx, PAy = (real)P.PAy, PAz = (real)P.PAz;             \
+    const real QCx = (real)Q.PAx, QCy = (real)Q.PAy, QCz = (real)Q.PAz;             \
+    const real oo2p = (real)P.inv2p, oo2q = (real)Q.inv2p, oo2pq = (real)0.5 * ipq; \
     (void)WPx; (void)WPy; (void)WPz; (void)WQx; (void)WQy; (void)WQz;               \
     (void)PAx; (void)PAy; (void)PAz; (void)QCx; (void)QCy; (void)QCz;               \
     (void)oo2p; (void)oo2q; (void)oo2pq; (void)rho_p; (void)rho_q;

@@ -186,9 +186,9 @@ def gen_eri_class(la, lb, lc, ld):
     for n in acc_names.values():
         lines.append("  real %s = 0;" % n)
     lines.append("  for (int i = 0; i < bh.n; ++i) {")
-    lines.append("    const PrimPair<real> P = args.prims[bh.off + i];")
+    lines.append("    const PrimPair<double> P = args.prims[bh.off + i];")
     lines.append("    for (int j = 0; j < kh.n; ++j) {")
-    lines.append("      const PrimPair<real> Q = args.prims[kh.off + j];")
+    lines.append("      const PrimPair<double> Q = args.prims[kh.off + j];")
     lines.append("      ERI_PRIM_SETUP(%d)" % L)
     lines.extend(body)
     lines.append("    }")

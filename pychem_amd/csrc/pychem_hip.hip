@@ -169,7 +169,7 @@ struct pq_basis {
     std::vector<std::vector<int>> class_pairs;   // pair ids per pair class
     std::vector<double> bound;              // per pair Schwarz bound
     int have_bound = 0;
-    DevBuf d_pairs, d_prims, d_prims32, d_atoms, d_cart_off, d_bound, d_tmp_list, d_spher;
+    DevBuf d_pairs, d_prims, d_atoms, d_cart_off, d_bound, d_tmp_list, d_spher;
     int any_spher = 0;
     EriClassStore store64[ERI_NCLASS], store32[ERI_NCLASS];
     double threshold = 1e-8, threshold32 = 0.0;
@@ -294,16 +294,9 @@ extern "C" int pq_basis_create(int nshell, const int* l, const int* nprim, const
         }
     b->npair = (int)b->pairs.size();
     b->nprimpair = (int)b->prims.size();
-    std::vector<PrimPair<float>> p32(b->prims.size());
-    for (size_t i = 0; i < b->prims.size(); ++i) {
-        const PrimPair<double>& s = b->prims[i];
-        p32[i] = {(float)s.Px, (float)s.Py, (float)s.Pz, (float)s.PAx, (float)s.PAy, (float)s.PAz,
-                  (float)s.p, (float)s.inv2p, (float)s.K, (float)s.beta};
-    }
     int rc = 0;
     rc |= b->d_pairs.ensure(sizeof(PairHdr) * b->pairs.size());
     rc |= b->d_prims.ensure(sizeof(PrimPair<double>) * std::max<size_t>(1, b->prims.size()));
-    rc |= b->d_prims32.ensure(sizeof(PrimPair<float>) * std::max<size_t>(1, b->prims.size()));
     rc |= b->d_atoms.ensure(sizeof(double) * b->atoms.size());
     rc |= b->d_cart_off.ensure(sizeof(int) * nshell);
     rc |= b->d_bound.ensure(sizeof(double) * b->pairs.size());
@@ -312,8 +305,6 @@ extern "C" int pq_basis_create(int nshell, const int* l, const int* nprim, const
     e = hipMemcpy(b->d_pairs.p, b->pairs.data(), sizeof(PairHdr) * b->pairs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !b->prims.empty())
         e = hipMemcpy(b->d_prims.p, b->prims.data(), sizeof(PrimPair<double>) * b->prims.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !p32.empty())
-        e = hipMemcpy(b->d_prims32.p, p32.data(), sizeof(PrimPair<float>) * p32.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->d_atoms.p, b->atoms.data(), sizeof(double) * b->atoms.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->d_cart_off.p, b->cart_off.data(), sizeof(int) * nshell, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -335,7 +326,7 @@ extern "C" void pq_basis_destroy(pq_basis* b)
         b->store64[c].q.release(); b->store64[c].vals.release(); b->store64[c].items.release();
         b->store32[c].q.release(); b->store32[c].vals.release(); b->store32[c].items.release();
     }
-    b->d_pairs.release(); b->d_prims.release(); b->d_prims32.release(); b->d_atoms.release();
+    b->d_pairs.release(); b->d_prims.release(); b->d_atoms.release();
     b->d_cart_off.release(); b->d_bound.release(); b->d_tmp_list.release(); b->d_spher.release();
     if (b->ev_jk[0]) {
         for (int i = 0; i < 2 * ERI_NCLASS; ++i)
@@ -417,7 +408,7 @@ static int run_class(pq_basis* b, int cls, const int2* dq, int nq, real* dout, h
     EriArgs<real> a;
     a.quartets = dq;
     a.pairs = (const PairHdr*)b->d_pairs.p;
-    a.prims = (const PrimPair<real>*)(sizeof(real) == 8 ? b->d_prims.p : b->d_prims32.p);
+    a.prims = (const PrimPair<double>*)b->d_prims.p;
     a.out = dout;
     a.nq = nq;
     eri_launch<real>(cls, a, st);

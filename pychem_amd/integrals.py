@@ -1,0 +1,56 @@
+"""The reference's integral-driver call surface (Methods/integrals.py),
+backed by the HIP engine.
+
+* ``nuclear_repulsion(molecule)``           Methods/integrals.py:196-207
+* ``one_electron(molecule, shell_pair)``     Methods/integrals.py:220-370
+  -> (core[nlA, nlB], overlap[nlA, nlB]) in the final (spherical or
+  Cartesian) basis, core = T + V summed over all atoms
+* ``two_electron(sp1, sp2, ints_type, grid_value)`` Methods/integrals.py:427-555
+  -> ndarray (nlA, nlB, nlC, nlD), normalised, chemists' notation
+
+These per-pair / per-quartet calls exist for drop-in compatibility; they
+launch tiny batches.  The SCF path (hartree_fock.py) uses the batched,
+device-resident entry points instead.
+"""
+import numpy as np
+
+from . import hartree_fock
+
+
+def nuclear_repulsion(molecule):
+    return hartree_fock.nuclear_repulsion(molecule)
+
+
+def _engine(molecule):
+    return hartree_fock.get_engine(molecule)
+
+
+def one_electron(molecule, shell_pair):
+    """Core (T+V) and overlap blocks of one shell pair."""
+    cache = getattr(molecule, "_onee_cache", None)
+    if cache is None:
+        S, T, V = _engine(molecule).one_electron()
+        cache = ((T + V).cpu().numpy(), S.cpu().numpy())
+        molecule._onee_cache = cache
+    core, ovl = cache
+    ia, ib = shell_pair.Centre1.Ivec, shell_pair.Centre2.Ivec
+    return core[np.ix_(ia, ib)], ovl[np.ix_(ia, ib)]
+
+
+def two_electron(shell_pair1, shell_pair2, ints_type=0, grid_value=-1.0):
+    """(ab|cd) block for shells (Centre1, Centre2 | Centre1, Centre2).
+
+    ``ints_type`` 0 is the Coulomb operator; the scattering operator
+    (ints_type 1, Methods/c_ints/two_electron_scattering.c) is not part of
+    this engine."""
+    if ints_type != 0:
+        raise NotImplementedError("scattering integrals (ints_type=1) are not implemented by the HIP engine")
+    mol = getattr(shell_pair1, "Molecule", None)
+    if mol is None:
+        raise ValueError("shell pair is not bound to a molecule; use molecule.ShellPairs[(a, b)]")
+    eng = _engine(mol)
+    a, b = shell_pair1.Centre1.Index, shell_pair1.Centre2.Index
+    c, d = shell_pair2.Centre1.Index, shell_pair2.Centre2.Index
+    blk = eng.eri_blocks([(a, b, c, d)])[0]
+    T = [mol.Shells[s].Cgtf.CartToSpher for s in (a, b, c, d)]
+    return np.einsum("ia,jb,kc,ld,abcd->ijkl", T[0], T[1], T[2], T[3], blk, optimize=True)

@@ -202,6 +202,7 @@ class Molecule:
             for b in range(a, len(self.Shells)):
                 sb = self.Shells[b]
                 sp[(a, b)] = ShellPair(sa.Coords, sa.Cgtf, a, sa.Ivec, sb.Coords, sb.Cgtf, b, sb.Ivec)
+                sp[(a, b)].Molecule = self
         self._shell_pairs = sp
         return sp
 

@@ -103,3 +103,27 @@ def test_jk_vs_dense_einsum(gpu):
     Jo, Ko = cint.jk(eri, Dt, D)
     assert np.abs(J.cpu().numpy() - Jo).max() < 1e-10
     assert np.abs(K.cpu().numpy() - Ko).max() < 1e-10
+
+
+def test_level_b_surface_matches_oracle(gpu):
+    """integrals.one_electron / two_electron (the reference's Methods/integrals.py
+    call surface) against the oracle, spherical d included."""
+    from pychem_amd import integrals
+    m = _mol(geometry.small_molecule("CH3OH", 0.05, 12), "6-31G*", cart=())
+    So, To, Vo = cint.one_electron(m)
+    sp = m.ShellPairs
+    rng = np.random.default_rng(4)
+    keys = list(sp.keys())
+    for k in rng.choice(len(keys), 20, replace=False):
+        a, b = keys[k]
+        core, ovl = integrals.one_electron(m, sp[(a, b)])
+        ia, ib = m.Shells[a].Ivec, m.Shells[b].Ivec
+        assert np.abs(ovl - So[np.ix_(ia, ib)]).max() < 1e-10
+        assert np.abs(core - (To + Vo)[np.ix_(ia, ib)]).max() < 1e-10
+    for _ in range(30):
+        (a, b), (c, d) = keys[rng.integers(len(keys))], keys[rng.integers(len(keys))]
+        blk = integrals.two_electron(sp[(a, b)], sp[(c, d)], 0, -1.0)
+        assert np.abs(blk - cint.eri_quartet(m, a, b, c, d)).max() < 1e-10
+    assert abs(integrals.nuclear_repulsion(m) - sum(
+        ai.NuclearCharge * aj.NuclearCharge / np.linalg.norm(np.subtract(ai.Coordinates, aj.Coordinates))
+        for i, ai in enumerate(m.Atoms) for aj in m.Atoms[:i])) < 1e-12
