@@ -96,6 +96,20 @@ int pq_eri_stats(const pq_basis* b, long long* stats, int nstats);
 int pq_jk(pq_basis* b, const double* Dj_dev, const double* Dk_dev, double* J_dev, double* K_dev,
           void* stream);
 
+/* Autotuning hooks of the J/K digestion: per class, work-item length
+ * (run_len, 0 = automatic) and number of blocks (grid, 0 = one per item,
+ * smaller = persistent blocks looping over items); and a single-class digest
+ * into unsymmetrised J', K' (nitems receives the work-item count). */
+int pq_jk_set_params(pq_basis* b, int cls, int run_len, int grid, void* stream);
+int pq_jk_class(pq_basis* b, int cls, const double* Dj_dev, const double* Dk_dev, double* J_dev, double* K_dev,
+                int* nitems, void* stream);
+
+/* pq_jk captured once into a HIP graph for fixed device buffers (kernel
+ * nodes only), and replayed on `stream`: the per-iteration Fock build of
+ * batched SCF without per-kernel launch overhead. */
+int pq_jk_graph_create(pq_basis* b, const double* Dj_dev, const double* Dk_dev, double* J_dev, double* K_dev);
+int pq_jk_graph_launch(pq_basis* b, void* stream);
+
 /* Dense (ab|cd) tensor [ncart]^4 from the stored ERIs (screened entries
  * zero), the CoulombIntegrals array of Util/structures.py:506 as filled by
  * Methods/hartree_fock.py:266-325. */

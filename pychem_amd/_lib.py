@@ -24,6 +24,13 @@ SIGNATURES = {
     "pq_basis_create": (c_int, [c_int, P_int, P_int, P_int, P_double, P_double, P_double,
                                 c_int, P_double, P_double, ctypes.POINTER(c_void_p)]),
     "pq_basis_destroy": (None, [c_void_p]),
+    "pq_basis_create_batch": (c_int, [c_int, c_int, P_int, P_int, P_int, P_double, P_double, P_double, P_int,
+                                      c_int, P_double, P_double, P_int, ctypes.POINTER(c_void_p)]),
+    "pq_basis_molecules": (c_int, [c_void_p, P_ll, P_int, P_int, P_int]),
+    "pq_jk_set_params": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
+    "pq_jk_class": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, P_int, c_void_p]),
+    "pq_jk_graph_create": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pq_jk_graph_launch": (c_int, [c_void_p, c_void_p]),
     "pq_basis_set_spherical": (c_int, [c_void_p, P_int]),
     "pq_basis_info": (c_int, [c_void_p, P_ll]),
     "pq_one_electron": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -1,0 +1,266 @@
+"""Batched RHF for many small molecules of uneven size on one GPU.
+
+All molecules share one engine handle (pq_basis_create_batch): one Schwarz
+pass, one ERI pass (21 class launches for the whole batch) and, per SCF
+iteration, one J/K digestion replayed from a captured HIP graph.  The host
+side works on padded per-molecule tensors [B, nmax, nmax] on the GPU: Fock
+assembly, Pulay DIIS (last 8 residuals), batched eigh, densities and energies
+for every molecule at once; converged molecules are frozen.
+
+The reference runs one molecule per process and has no batched path; the
+per-molecule algorithm is the same RHF as hartree_fock.py (closed shells,
+Cartesian or spherical basis as given).
+"""
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib, structures
+from ._lib import check, dptr, iptr
+from .engine import _ptr, _stream
+
+
+class BatchEngine:
+    """One device handle over many molecules (packed per-molecule matrices)."""
+
+    def __init__(self, mols, device="cuda"):
+        if not torch.cuda.is_available():
+            raise _lib.HipError("no GPU visible: the HIP engine has no CPU fallback")
+        self.mols = mols
+        self.device = torch.device(device)
+        tabs = [m.shell_table() for m in mols]
+        for t in tabs:
+            if t["nao"] != t["ncart_total"]:
+                raise ValueError("batched path expects Cartesian-basis molecules (cartesian_l covering d)")
+        l = np.concatenate([t["l"] for t in tabs]).astype(np.int32)
+        nprim = np.concatenate([t["nprim"] for t in tabs]).astype(np.int32)
+        prim_off = np.zeros(len(l), dtype=np.int32)
+        prim_off[1:] = np.cumsum(nprim)[:-1]
+        center = np.concatenate([t["center"] for t in tabs]).astype(float)
+        exps = np.concatenate([t["exps"] for t in tabs]).astype(float)
+        cc = np.concatenate([t["cc"] for t in tabs]).astype(float)
+        shell_mol = np.concatenate([np.full(t["nshell"], i, dtype=np.int32) for i, t in enumerate(tabs)])
+        axyz = np.concatenate([t["atom_xyz"] for t in tabs]).astype(float)
+        az = np.concatenate([t["atom_z"] for t in tabs]).astype(float)
+        amol = np.concatenate([np.full(t["natom"], i, dtype=np.int32) for i, t in enumerate(tabs)])
+        self._keep = (l, nprim, prim_off, center, exps, cc, shell_mol, axyz, az, amol)
+        h = ctypes.c_void_p()
+        L = _lib.lib()
+        check(L.pq_basis_create_batch(len(mols), len(l), iptr(l), iptr(nprim), iptr(prim_off), dptr(center),
+                                      dptr(exps), dptr(cc), iptr(shell_mol), len(az), dptr(axyz), dptr(az),
+                                      iptr(amol), ctypes.byref(h)), "pq_basis_create_batch")
+        self.h = h
+        info = np.zeros(7, dtype=np.int64)
+        check(L.pq_basis_info(h, info.ctypes.data_as(_lib.P_ll)), "pq_basis_info")
+        self.nmol, self.mat_total, self.nmax = int(info[4]), int(info[5]), int(info[6])
+        self.base = np.zeros(self.nmol, dtype=np.int64)
+        self.n = np.zeros(self.nmol, dtype=np.int32)
+        check(L.pq_basis_molecules(h, self.base.ctypes.data_as(_lib.P_ll), iptr(self.n), None, None),
+              "pq_basis_molecules")
+        # packed position -> padded flat position [B, nmax, nmax]
+        idx = []
+        nm = self.nmax
+        for m in range(self.nmol):
+            n = int(self.n[m])
+            r = np.arange(n)
+            idx.append((m * nm * nm + r[:, None] * nm + r[None, :]).reshape(-1))
+        self.pidx = torch.tensor(np.concatenate(idx), device=self.device)
+        self.graph = False
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().pq_basis_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def to_padded(self, packed):
+        out = torch.zeros(self.nmol * self.nmax * self.nmax, dtype=packed.dtype, device=self.device)
+        out[self.pidx] = packed
+        return out.view(self.nmol, self.nmax, self.nmax)
+
+    def to_packed(self, padded, out=None):
+        flat = padded.reshape(-1)[self.pidx]
+        if out is None:
+            return flat
+        out.copy_(flat)
+        return out
+
+    def one_electron(self):
+        S = torch.empty(self.mat_total, dtype=torch.float64, device=self.device)
+        T = torch.empty_like(S)
+        V = torch.empty_like(S)
+        check(_lib.lib().pq_one_electron(self.h, _ptr(S), _ptr(T), _ptr(V), _stream()), "pq_one_electron")
+        return S, T, V
+
+    def build_eri(self, threshold=1e-8, fp32_threshold=0.0):
+        check(_lib.lib().pq_eri_build(self.h, float(threshold), float(fp32_threshold), _stream()), "pq_eri_build")
+
+    def stats(self):
+        nc = _lib.lib().pq_eri_nclass()
+        st = np.zeros(nc * 6, dtype=np.int64)
+        check(_lib.lib().pq_eri_stats(self.h, st.ctypes.data_as(_lib.P_ll), nc * 6), "pq_eri_stats")
+        return st.reshape(nc, 6)
+
+    def jk_buffers(self):
+        """Fixed device buffers used by the captured J/K graph."""
+        self.Dj = torch.zeros(self.mat_total, dtype=torch.float64, device=self.device)
+        self.Dk = torch.zeros_like(self.Dj)
+        self.J = torch.zeros_like(self.Dj)
+        self.K = torch.zeros_like(self.Dj)
+        torch.cuda.synchronize()
+        check(_lib.lib().pq_jk_graph_create(self.h, _ptr(self.Dj), _ptr(self.Dk), _ptr(self.J), _ptr(self.K)),
+              "pq_jk_graph_create")
+        self.graph = True
+
+    def jk_replay(self):
+        check(_lib.lib().pq_jk_graph_launch(self.h, _stream()), "pq_jk_graph_launch")
+
+    def jk_direct(self):
+        check(_lib.lib().pq_jk(self.h, _ptr(self.Dj), _ptr(self.Dk), _ptr(self.J), _ptr(self.K), _stream()), "pq_jk")
+
+
+def _sad_padded(eng, mols):
+    from .guess import sad_density
+    B, nm = eng.nmol, eng.nmax
+    D = torch.zeros((B, nm, nm), dtype=torch.float64)
+    for m, mol in enumerate(mols):
+        d = torch.as_tensor(sad_density(mol))
+        D[m, :d.shape[0], :d.shape[0]] = d
+    return D.to(eng.device)
+
+
+def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_graph=True, diis_size=8):
+    """Closed-shell RHF for every molecule in ``mols`` at once.
+
+    Returns dict: energies [B], iterations [B], converged [B], per-molecule
+    latency (seconds from start until its convergence), stage timings."""
+    t_start = time.perf_counter()
+    for m in mols:
+        if m.NAlphaElectrons != m.NBetaElectrons:
+            raise ValueError("rhf_batch handles closed-shell molecules")
+    eng = BatchEngine(mols)
+    dev = eng.device
+    B, nm = eng.nmol, eng.nmax
+    S, T, V = eng.one_electron()
+    eng.build_eri(threshold)
+    Sp = eng.to_padded(S)
+    Hp = eng.to_padded(T + V)
+    n_t = torch.as_tensor(eng.n, device=dev, dtype=torch.long)
+    ar = torch.arange(nm, device=dev)
+    valid = (ar[None, :] < n_t[:, None])                                 # [B, nm]
+    vmask = (valid[:, :, None] & valid[:, None, :]).to(torch.float64)
+    # canonical orthogonaliser per molecule; the pad block of S is 1e3 * I so
+    # its eigenvalues separate from the molecular ones and are dropped
+    pad_eye = 1e3 * torch.diag_embed((~valid).to(torch.float64))
+    s, U = torch.linalg.eigh(Sp + pad_eye)
+    keep = (s > structures.LINEAR_DEPENDENCE) & (s < 5e2)               # [B, nm] kept columns
+    X = U * torch.where(keep, s.clamp_min(1e-300) ** -0.5, torch.zeros_like(s))[:, None, :]
+    X = X * valid[:, :, None]
+    big = 1e6 * torch.diag_embed((~keep).to(torch.float64))             # pushes dropped columns up
+    nocc = torch.as_tensor([m.NAlphaElectrons for m in mols], device=dev)
+    occ = (ar[None, :] < nocc[:, None]).to(torch.float64)                # [B, nm]
+    enuc = torch.as_tensor([structures_nuclear(m) for m in mols], device=dev)
+    if guess == "SAD":
+        D = 0.5 * _sad_padded(eng, mols)
+    else:
+        M = X.transpose(1, 2) @ Hp @ X + big
+        e, C = torch.linalg.eigh(M)
+        C = X @ C
+        D = (C * occ[:, None, :]) @ C.transpose(1, 2)
+    if use_graph:
+        eng.jk_buffers()
+    else:
+        eng.Dj = torch.zeros(eng.mat_total, dtype=torch.float64, device=dev)
+        eng.Dk = torch.zeros_like(eng.Dj)
+        eng.J = torch.zeros_like(eng.Dj)
+        eng.K = torch.zeros_like(eng.Dj)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_start
+    E = torch.zeros(B, dtype=torch.float64, device=dev)
+    done = torch.zeros(B, dtype=torch.bool, device=dev)
+    iters = torch.zeros(B, dtype=torch.long, device=dev)
+    latency = np.full(B, np.nan)
+    Fh, Rh = [], []
+    for it in range(1, maxiter + 1):
+        eng.to_packed(2.0 * D, out=eng.Dj)
+        eng.to_packed(D, out=eng.Dk)
+        if use_graph:
+            eng.jk_replay()
+        else:
+            eng.jk_direct()
+        F = Hp + eng.to_padded(eng.J) + eng.to_padded(eng.K)
+        # Pulay DIIS on X^T (S D F - F D S) X, batched
+        R = X.transpose(1, 2) @ (Sp @ D @ F - F @ D @ Sp) @ X
+        Fh.append(F)
+        Rh.append(R)
+        if len(Fh) > diis_size:
+            Fh.pop(0)
+            Rh.pop(0)
+        k = len(Fh)
+        if k >= 2:
+            Rs = torch.stack(Rh, 1)                                      # [B, k, nm, nm]
+            Bm = torch.einsum("bkij,blij->bkl", Rs, Rs)
+            A = torch.full((B, k + 1, k + 1), -1.0, dtype=torch.float64, device=dev)
+            A[:, :k, :k] = Bm
+            A[:, k, k] = 0.0
+            # tiny ridge keeps the bordered system regular for converged molecules
+            A[:, :k, :k] += torch.diag_embed((1e-14 * Bm.diagonal(dim1=1, dim2=2).mean(1, keepdim=True)
+                                              + 1e-300).expand(B, k))
+            rhs = torch.zeros((B, k + 1, 1), dtype=torch.float64, device=dev)
+            rhs[:, k, 0] = -1.0
+            c = torch.linalg.solve(A, rhs)[:, :k, 0]                     # [B, k]
+            Fx = torch.einsum("bk,bkij->bij", c, torch.stack(Fh, 1))
+        else:
+            Fx = F
+        M = X.transpose(1, 2) @ Fx @ X + big
+        e, C = torch.linalg.eigh(M)
+        C = X @ C
+        Dn = (C * occ[:, None, :]) @ C.transpose(1, 2)
+        En = ((Dn * (Hp + Fx)) * vmask).sum((1, 2)) + enuc
+        newly = (~done) & ((En - E).abs() < conv) & (it > 1)
+        D = torch.where(done[:, None, None], D, Dn)
+        E = torch.where(done, E, En)
+        iters = torch.where(done, iters, torch.full_like(iters, it))
+        if bool(newly.any()):
+            tnow = time.perf_counter() - t_start
+            for m in torch.nonzero(newly).flatten().tolist():
+                latency[m] = tnow
+        done = done | newly
+        if bool(done.all()):
+            break
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t_start
+    latency[np.isnan(latency)] = total
+    out = dict(energies=E.cpu().numpy(), iterations=iters.cpu().numpy(), converged=done.cpu().numpy(),
+               latency_s=latency, total_s=total, setup_s=t_setup, nmol=B, stats=eng.stats())
+    eng.close()
+    return out
+
+
+def structures_nuclear(mol):
+    from .hartree_fock import nuclear_repulsion
+    return nuclear_repulsion(mol)
+
+
+def config4_molecules(count=512, seed=4, sigma=0.05):
+    """512 molecules drawn by numpy.random.default_rng(seed) from
+    {H2O, NH3, CH4, C2H6, CH3OH, C3H8} x {STO-3G, 6-31G*}, coordinate jitter
+    sigma (Angstrom); 6-31G* with Cartesian d."""
+    from . import geometry
+    rng = np.random.default_rng(seed)
+    names = ["H2O", "NH3", "CH4", "C2H6", "CH3OH", "C3H8"]
+    bases = ["STO-3G", "6-31G*"]
+    mols = []
+    for k in range(count):
+        name = names[int(rng.integers(len(names)))]
+        basis = bases[int(rng.integers(len(bases)))]
+        rows = geometry.small_molecule(name, sigma, int(rng.integers(1 << 30)))
+        mols.append(structures.Molecule(rows, basis=basis, cartesian_l=[2]))
+    return mols

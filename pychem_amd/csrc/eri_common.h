@@ -13,6 +13,16 @@ struct PairHdr {
     int n;          // primitive pairs kept
     int s1, s2;     // shells: primary, secondary
     int pcls;       // pair class 0..5: ss ps pp ds dp dd
+    int mol;        // molecule of both shells
+};
+
+// One molecule of a (possibly batched) basis: its matrices are the n x n
+// block at offset base of the packed matrix arrays; its atoms are
+// [atom_lo, atom_hi).  Shell AO offsets are local to the molecule.
+struct MolInfo {
+    long long base;
+    int n;
+    int atom_lo, atom_hi;
     int pad;
 };
 
@@ -36,8 +46,9 @@ struct OneeArgs {
     const PrimPair<double>* prims;
     const double* atoms;     // x, y, z, Z per atom
     const int* cart_off;
-    double *S, *T, *V;
-    int npair, natom, ncart;
+    const MolInfo* mols;
+    double *S, *T, *V;       // packed per-molecule blocks
+    int npair;
 };
 
 template <typename real>
@@ -45,6 +56,7 @@ struct JkArgs {
     const int2* quartets;
     const PairHdr* pairs;
     const int* cart_off;
+    const MolInfo* mols;
     const real* vals;
     const double* Dj;   // density for J (ncart x ncart)
     const double* Dk;   // density for K

@@ -295,6 +295,7 @@ def gen_onee(la, lb):
     lines.append("  if (t >= args.npair) return;")
     lines.append("  const int pidx = args.pair_list[t];")
     lines.append("  const PairHdr h = args.pairs[pidx];")
+    lines.append("  const MolInfo mi = args.mols[h.mol];")
     lines.append("  const double ABx = h.ab[0], ABy = h.ab[1], ABz = h.ab[2];")
     lines.append("  (void)ABx; (void)ABy; (void)ABz;")
     for a in comps(la):
@@ -305,7 +306,7 @@ def gen_onee(la, lb):
     lines.append("    const double PAx = P.PAx, PAy = P.PAy, PAz = P.PAz;")
     lines.append("    const double beta = P.beta;")
     lines.extend(sbody)
-    lines.append("    for (int c = 0; c < args.natom; ++c) {")
+    lines.append("    for (int c = mi.atom_lo; c < mi.atom_hi; ++c) {")
     lines.append("      const double PCx = P.Px - args.atoms[4 * c], PCy = P.Py - args.atoms[4 * c + 1], PCz = P.Pz - args.atoms[4 * c + 2];")
     lines.append("      const double Tn = P.p * (PCx * PCx + PCy * PCy + PCz * PCz);")
     lines.append("      double F[%d];" % (Lv + 1))
@@ -315,7 +316,10 @@ def gen_onee(la, lb):
     lines.append("    }")
     lines.append("  }")
     lines.append("  const int o1 = args.cart_off[h.s1], o2 = args.cart_off[h.s2];")
-    lines.append("  const int N = args.ncart;")
+    lines.append("  const int N = mi.n;")
+    lines.append("  double* const Sm = args.S + mi.base;")
+    lines.append("  double* const Tm = args.T + mi.base;")
+    lines.append("  double* const Vm = args.V + mi.base;")
     ia = 0
     for a in comps(la):
         ib = 0
@@ -323,7 +327,7 @@ def gen_onee(la, lb):
             nrm = cnorm(a) * cnorm(b)
             lines.append("  { const size_t u = (size_t)(o1 + %d) * N + (o2 + %d), w = (size_t)(o2 + %d) * N + (o1 + %d);" % (ia, ib, ib, ia))
             for M, v in (("S", "s"), ("T", "t"), ("V", "v")):
-                lines.append("    args.%s[u] = %s_%s_%s * %.17g; args.%s[w] = %s_%s_%s * %.17g;" %
+                lines.append("    %sm[u] = %s_%s_%s * %.17g; %sm[w] = %s_%s_%s * %.17g;" %
                              (M, v, nm(a), nm(b), nrm, M, v, nm(a), nm(b), nrm))
             lines.append("  }")
             ib += 1
@@ -363,7 +367,7 @@ def main():
             jsrc.append("void jk_run_%s_%s(const JkArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
             jsrc.append("{ hipLaunchKernelGGL((jk_kernel<%d, %d, %d, %d, %s>), dim3((a.nq + 127) / 128), dim3(128), 0, st, a); }" % (la, lb, lc, ld, rt))
             jsrc.append("void jkrun_run_%s_%s(const JkRunArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
-            jsrc.append("{ hipLaunchKernelGGL((jk_run_kernel<%d, %d, %d, %d, %s>), dim3(a.nitems), dim3(64), "
+            jsrc.append("{ hipLaunchKernelGGL((jk_run_kernel<%d, %d, %d, %d, %s>), dim3(a.grid), dim3(64), "
                         "sizeof(double) * %d * a.n, st, a); }" % (la, lb, lc, ld, rt, (la + 1) * (la + 2) // 2 + (lb + 1) * (lb + 2) // 2))
             jsrc.append("void scatter_run_%s_%s(const ScatterArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
             jsrc.append("{ hipLaunchKernelGGL((scatter_kernel<%d, %d, %d, %d, %s>), dim3((a.nq + 127) / 128), dim3(128), 0, st, a); }" % (la, lb, lc, ld, rt))

@@ -40,6 +40,7 @@ __global__ void __launch_bounds__(128) jk_kernel(const JkArgs<real> a)
     const PairHdr bh = a.pairs[pr.x], kh = a.pairs[pr.y];
     const int o1 = a.cart_off[bh.s1], o2 = a.cart_off[bh.s2];
     const int o3 = a.cart_off[kh.s1], o4 = a.cart_off[kh.s2];
+    const MolInfo mi = a.mols[bh.mol];
     double sc = valid ? 1.0 : 0.0;
     if (bh.s1 == bh.s2) sc *= 0.5;
     if (kh.s1 == kh.s2) sc *= 0.5;
@@ -47,9 +48,11 @@ __global__ void __launch_bounds__(128) jk_kernel(const JkArgs<real> a)
     const int bra0 = __shfl(pr.x, 0, 64);
     const bool uniform = __all(pr.x == bra0);
     const int lane = threadIdx.x & 63;
-    const int n = a.n;
-    const double* Dj = a.Dj;
-    const double* Dk = a.Dk;
+    const int n = mi.n;
+    const double* Dj = a.Dj + mi.base;
+    const double* Dk = a.Dk + mi.base;
+    double* const Jm = a.J + mi.base;
+    double* const Km = a.K + mi.base;
     const size_t nq = a.nq;
     double j34[N3 * N4], k23[N2 * N3], k24[N2 * N4];
 #pragma unroll
@@ -84,7 +87,7 @@ __global__ void __launch_bounds__(128) jk_kernel(const JkArgs<real> a)
                     k24[ib * N4 + id] += d13 * v;
                 }
             }
-            double* dst = &a.J[(size_t)(o1 + ia) * n + o2 + ib];
+            double* dst = &Jm[(size_t)(o1 + ia) * n + o2 + ib];
             if (uniform) {
                 const double s = wave_sum(4.0 * sc * j12);
                 if (lane == 0) atomic_addd(dst, s);
@@ -94,22 +97,22 @@ __global__ void __launch_bounds__(128) jk_kernel(const JkArgs<real> a)
         }
         if (valid) {
 #pragma unroll
-            for (int ic = 0; ic < N3; ++ic) atomic_addd(&a.K[(size_t)(o1 + ia) * n + o3 + ic], 2.0 * sc * k13[ic]);
+            for (int ic = 0; ic < N3; ++ic) atomic_addd(&Km[(size_t)(o1 + ia) * n + o3 + ic], 2.0 * sc * k13[ic]);
 #pragma unroll
-            for (int id = 0; id < N4; ++id) atomic_addd(&a.K[(size_t)(o1 + ia) * n + o4 + id], 2.0 * sc * k14[id]);
+            for (int id = 0; id < N4; ++id) atomic_addd(&Km[(size_t)(o1 + ia) * n + o4 + id], 2.0 * sc * k14[id]);
         }
     }
     if (!valid) return;
 #pragma unroll
     for (int ic = 0; ic < N3; ++ic)
 #pragma unroll
-        for (int id = 0; id < N4; ++id) atomic_addd(&a.J[(size_t)(o3 + ic) * n + o4 + id], 4.0 * sc * j34[ic * N4 + id]);
+        for (int id = 0; id < N4; ++id) atomic_addd(&Jm[(size_t)(o3 + ic) * n + o4 + id], 4.0 * sc * j34[ic * N4 + id]);
 #pragma unroll
     for (int ib = 0; ib < N2; ++ib) {
 #pragma unroll
-        for (int ic = 0; ic < N3; ++ic) atomic_addd(&a.K[(size_t)(o2 + ib) * n + o3 + ic], 2.0 * sc * k23[ib * N3 + ic]);
+        for (int ic = 0; ic < N3; ++ic) atomic_addd(&Km[(size_t)(o2 + ib) * n + o3 + ic], 2.0 * sc * k23[ib * N3 + ic]);
 #pragma unroll
-        for (int id = 0; id < N4; ++id) atomic_addd(&a.K[(size_t)(o2 + ib) * n + o4 + id], 2.0 * sc * k24[ib * N4 + id]);
+        for (int id = 0; id < N4; ++id) atomic_addd(&Km[(size_t)(o2 + ib) * n + o4 + id], 2.0 * sc * k24[ib * N4 + id]);
     }
 }
 
@@ -125,12 +128,13 @@ struct JkRunArgs {
     const int2* items;       // [start, end) per work item, one bra pair each
     const PairHdr* pairs;
     const int* cart_off;
+    const MolInfo* mols;
     const real* vals;
     const double* Dj;
     const double* Dk;
     double* J;
     double* K;
-    int nq, n, nitems;
+    int nq, n, nitems, grid;   // grid: blocks launched (<= nitems; persistent when smaller)
 };
 
 template <int LA, int LB, int LC, int LD, typename real>
@@ -139,20 +143,23 @@ __global__ void __launch_bounds__(64) jk_run_kernel(const JkRunArgs<real> a)
     constexpr int N1 = (LA + 1) * (LA + 2) / 2, N2 = (LB + 1) * (LB + 2) / 2;
     constexpr int N3 = (LC + 1) * (LC + 2) / 2, N4 = (LD + 1) * (LD + 2) / 2;
     extern __shared__ double krows[];   // (N1 + N2) x n
-    const int item = blockIdx.x;
-    if (item >= a.nitems) return;
+    // persistent: the grid may be smaller than the number of work items
+    for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
     const int2 rng = a.items[item];
     const int lane = threadIdx.x;
-    const int n = a.n;
+    const int2 pr0 = a.quartets[rng.x];
+    const PairHdr bh = a.pairs[pr0.x];
+    const MolInfo mi = a.mols[bh.mol];
+    const int n = mi.n;
     const int nrow = (N1 + N2) * n;
     for (int i = lane; i < nrow; i += 64) krows[i] = 0.0;
     __syncthreads();
-    const int2 pr0 = a.quartets[rng.x];
-    const PairHdr bh = a.pairs[pr0.x];
     const int o1 = a.cart_off[bh.s1], o2 = a.cart_off[bh.s2];
     const double sbra = (bh.s1 == bh.s2) ? 0.5 : 1.0;
-    const double* Dj = a.Dj;
-    const double* Dk = a.Dk;
+    const double* Dj = a.Dj + mi.base;
+    const double* Dk = a.Dk + mi.base;
+    double* const Jm = a.J + mi.base;
+    double* const Km = a.K + mi.base;
     const size_t nq = a.nq;
     double j12[N1 * N2];
 #pragma unroll
@@ -207,7 +214,7 @@ __global__ void __launch_bounds__(64) jk_run_kernel(const JkRunArgs<real> a)
 #pragma unroll
         for (int ic = 0; ic < N3; ++ic)
 #pragma unroll
-            for (int id = 0; id < N4; ++id) atomic_addd(&a.J[(size_t)(o3 + ic) * n + o4 + id], 4.0 * sc * j34[ic * N4 + id]);
+            for (int id = 0; id < N4; ++id) atomic_addd(&Jm[(size_t)(o3 + ic) * n + o4 + id], 4.0 * sc * j34[ic * N4 + id]);
 #pragma unroll
         for (int ib = 0; ib < N2; ++ib) {
 #pragma unroll
@@ -219,7 +226,7 @@ __global__ void __launch_bounds__(64) jk_run_kernel(const JkRunArgs<real> a)
 #pragma unroll
     for (int i = 0; i < N1 * N2; ++i) {
         const double s = wave_sum(4.0 * j12[i]);
-        if (lane == 0) atomic_addd(&a.J[(size_t)(o1 + i / N2) * n + o2 + i % N2], s);
+        if (lane == 0) atomic_addd(&Jm[(size_t)(o1 + i / N2) * n + o2 + i % N2], s);
     }
     __syncthreads();
     for (int i = lane; i < nrow; i += 64) {
@@ -227,8 +234,10 @@ __global__ void __launch_bounds__(64) jk_run_kernel(const JkRunArgs<real> a)
         if (v != 0.0) {
             const int r = i / n, c = i - r * n;
             const int row = r < N1 ? o1 + r : o2 + (r - N1);
-            atomic_addd(&a.K[(size_t)row * n + c], v);
+            atomic_addd(&Km[(size_t)row * n + c], v);
         }
+    }
+    __syncthreads();
     }
 }
 

@@ -41,14 +41,29 @@ __global__ void boys_test_kernel(const double* T, int n, int mmax, double* out)
 }
 
 
-// J = (J' + J'^T)/2 ; K = -(K' + K'^T)/2  (reference sign convention for K)
-__global__ void jk_finish_kernel(double* J, double* K, int n)
+// Zero fill (used instead of hipMemsetAsync so that a captured graph holds
+// only kernel nodes).
+__global__ void fill_zero_kernel(double* a, long long n, double* b)
 {
-    const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n || j < i) return;
-    const size_t u = (size_t)i * n + j, w = (size_t)j * n + i;
-    if (J) { const double s = 0.5 * (J[u] + J[w]); J[u] = s; J[w] = s; }
-    if (K) { const double s = -0.5 * (K[u] + K[w]); K[u] = s; K[w] = s; }
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        a[i] = 0.0;
+        if (b) b[i] = 0.0;
+    }
+}
+
+// J = (J' + J'^T)/2 ; K = -(K' + K'^T)/2 per molecule block (reference sign
+// convention for K).  blockIdx.y = molecule.
+__global__ void jk_finish_kernel(double* J, double* K, const MolInfo* mols)
+{
+    const MolInfo mi = mols[blockIdx.y];
+    const long long nn = (long long)mi.n * mi.n;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nn; t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t / mi.n), j = (int)(t - (long long)i * mi.n);
+        if (j < i) continue;
+        const long long u = mi.base + t, w = mi.base + (long long)j * mi.n + i;
+        if (J) { const double s = 0.5 * (J[u] + J[w]); J[u] = s; J[w] = s; }
+        if (K) { const double s = -0.5 * (K[u] + K[w]); K[u] = s; K[w] = s; }
+    }
 }
 
 // Schwarz: per pair, max over its final-basis functions (m,n) of
@@ -153,6 +168,8 @@ struct EriClassStore {
     std::vector<int2> hitems;  // J/K work items: [start, end) ranges with one bra pair
     DevBuf q, vals, items;
     int nitems = 0;
+    int jk_run = 0;            // J/K work-item length (0: automatic)
+    int jk_grid = 0;           // J/K blocks launched (0: one per item)
     int nq = 0;
     int ncomp = 0;
     int is32 = 0;              // values stored as float
@@ -161,7 +178,11 @@ struct EriClassStore {
 
 struct pq_basis {
     int nshell = 0, natom = 0, ncart = 0, npair = 0, nprimpair = 0;
-    std::vector<int> l, nprim, cart_off;
+    int nmol = 1, nmax = 0;                 // molecules; largest per-molecule ncart
+    long long mat_total = 0;                // packed matrix elements, sum of n_m^2
+    std::vector<MolInfo> mols;
+    std::vector<int> shell_mol;
+    std::vector<int> l, nprim, cart_off;    // cart_off: local to the shell's molecule
     std::vector<double> center, atoms;
     std::vector<PairHdr> pairs;
     std::vector<PrimPair<double>> prims;
@@ -169,7 +190,7 @@ struct pq_basis {
     std::vector<std::vector<int>> class_pairs;   // pair ids per pair class
     std::vector<double> bound;              // per pair Schwarz bound
     int have_bound = 0;
-    DevBuf d_pairs, d_prims, d_atoms, d_cart_off, d_bound, d_tmp_list, d_spher;
+    DevBuf d_pairs, d_prims, d_atoms, d_cart_off, d_bound, d_tmp_list, d_spher, d_mols;
     int any_spher = 0;
     EriClassStore store64[ERI_NCLASS], store32[ERI_NCLASS];
     double threshold = 1e-8, threshold32 = 0.0;
@@ -179,6 +200,7 @@ struct pq_basis {
     int ev_cls_used[2 * ERI_NCLASS] = {};
     hipEvent_t ev_jk[2] = {};
     int ev_jk_used = 0;
+    hipGraphExec_t graph_exec = nullptr;
 };
 
 static int ensure_events(pq_basis* b)
@@ -212,49 +234,84 @@ extern "C" int pq_boys(int n, const double* T_dev, int mmax, double* out_dev, vo
     return 0;
 }
 
-extern "C" int pq_basis_create(int nshell, const int* l, const int* nprim, const int* prim_off, const double* center,
-                               const double* exps, const double* cc, int natom, const double* atom_xyz,
-                               const double* atom_z, pq_basis** out)
+extern "C" void pq_basis_destroy(pq_basis* b);
+
+// Batched basis: nmol molecules, shells and atoms grouped by molecule
+// (shell_mol / atom_mol non-decreasing).  Pairs and quartets never mix
+// molecules; matrices are packed per-molecule blocks.
+extern "C" int pq_basis_create_batch(int nmol, int nshell, const int* l, const int* nprim, const int* prim_off,
+                                     const double* center, const double* exps, const double* cc,
+                                     const int* shell_mol, int natom, const double* atom_xyz, const double* atom_z,
+                                     const int* atom_mol, pq_basis** out)
 {
     if (!out) return fail("pq_basis_create: null output");
     *out = nullptr;
-    if (nshell <= 0) return fail("pq_basis_create: nshell must be positive");
+    if (nshell <= 0 || nmol <= 0) return fail("pq_basis_create: nshell and nmol must be positive");
     for (int s = 0; s < nshell; ++s) {
         if (l[s] < 0 || l[s] > 2) return fail("pq_basis_create: only s, p, d shells are supported");
         if (nprim[s] <= 0 || nprim[s] > 64) return fail("pq_basis_create: bad primitive count");
+        const int m = shell_mol ? shell_mol[s] : 0;
+        if (m < 0 || m >= nmol || (s > 0 && shell_mol && shell_mol[s] < shell_mol[s - 1]))
+            return fail("pq_basis_create: shells must be grouped by molecule");
+    }
+    for (int a = 0; a < natom; ++a) {
+        const int m = atom_mol ? atom_mol[a] : 0;
+        if (m < 0 || m >= nmol || (a > 0 && atom_mol && atom_mol[a] < atom_mol[a - 1]))
+            return fail("pq_basis_create: atoms must be grouped by molecule");
     }
     pq_basis* b = new pq_basis();
     b->nshell = nshell;
     b->natom = natom;
+    b->nmol = nmol;
     b->l.assign(l, l + nshell);
     b->nprim.assign(nprim, nprim + nshell);
     b->center.assign(center, center + 3 * nshell);
+    b->shell_mol.resize(nshell);
+    for (int s = 0; s < nshell; ++s) b->shell_mol[s] = shell_mol ? shell_mol[s] : 0;
     b->cart_off.resize(nshell);
-    int off = 0;
+    b->mols.assign(nmol, MolInfo{0, 0, 0, 0, 0});
     for (int s = 0; s < nshell; ++s) {
-        b->cart_off[s] = off;
-        off += ncart_l(l[s]);
+        MolInfo& mi = b->mols[b->shell_mol[s]];
+        b->cart_off[s] = mi.n;
+        mi.n += ncart_l(l[s]);
     }
-    b->ncart = off;
+    for (int m = 0; m < nmol; ++m) { b->mols[m].atom_lo = natom; b->mols[m].atom_hi = 0; }
+    for (int a = 0; a < natom; ++a) {
+        MolInfo& mi = b->mols[atom_mol ? atom_mol[a] : 0];
+        mi.atom_lo = std::min(mi.atom_lo, a);
+        mi.atom_hi = std::max(mi.atom_hi, a + 1);
+    }
+    long long base = 0;
+    b->ncart = 0;
+    for (int m = 0; m < nmol; ++m) {
+        MolInfo& mi = b->mols[m];
+        if (mi.atom_hi <= mi.atom_lo) { mi.atom_lo = 0; mi.atom_hi = 0; }
+        mi.base = base;
+        base += (long long)mi.n * mi.n;
+        b->ncart += mi.n;
+        b->nmax = std::max(b->nmax, mi.n);
+    }
+    b->mat_total = base;
     b->atoms.resize(4 * (size_t)std::max(natom, 1));
     for (int a = 0; a < natom; ++a) {
         for (int i = 0; i < 3; ++i) b->atoms[4 * a + i] = atom_xyz[3 * a + i];
         b->atoms[4 * a + 3] = atom_z[a];
     }
-    // shell pairs a <= b, primary = larger l (first on ties); primitive-pair
-    // quantities as Methods/c_ints/shellpair_quantities.c:28-33
+    // shell pairs a <= b within a molecule, primary = larger l (first on ties);
+    // primitive-pair quantities as Methods/c_ints/shellpair_quantities.c:28-33
     b->pair_index.assign((size_t)nshell * nshell, -1);
     b->class_pairs.assign(6, {});
     const double PI = 3.14159265358979323846;
     for (int sa = 0; sa < nshell; ++sa)
         for (int sb = sa; sb < nshell; ++sb) {
+            if (b->shell_mol[sb] != b->shell_mol[sa]) break;   // grouped: later shells are other molecules
             PairHdr h;
             int s1 = sa, s2 = sb;
             if (l[sb] > l[sa]) { s1 = sb; s2 = sa; }
             h.s1 = s1;
             h.s2 = s2;
             h.pcls = pair_class(l[s1], l[s2]);
-            h.pad = 0;
+            h.mol = b->shell_mol[sa];
             double AB[3], ab2 = 0.0;
             for (int i = 0; i < 3; ++i) {
                 AB[i] = center[3 * s1 + i] - center[3 * s2 + i];
@@ -300,15 +357,17 @@ extern "C" int pq_basis_create(int nshell, const int* l, const int* nprim, const
     rc |= b->d_atoms.ensure(sizeof(double) * b->atoms.size());
     rc |= b->d_cart_off.ensure(sizeof(int) * nshell);
     rc |= b->d_bound.ensure(sizeof(double) * b->pairs.size());
-    if (rc) { delete b; return -1; }
+    rc |= b->d_mols.ensure(sizeof(MolInfo) * nmol);
+    if (rc) { pq_basis_destroy(b); return -1; }
     hipError_t e = hipSuccess;
     e = hipMemcpy(b->d_pairs.p, b->pairs.data(), sizeof(PairHdr) * b->pairs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !b->prims.empty())
         e = hipMemcpy(b->d_prims.p, b->prims.data(), sizeof(PrimPair<double>) * b->prims.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->d_atoms.p, b->atoms.data(), sizeof(double) * b->atoms.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->d_cart_off.p, b->cart_off.data(), sizeof(int) * nshell, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_mols.p, b->mols.data(), sizeof(MolInfo) * nmol, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        delete b;
+        pq_basis_destroy(b);
         return fail(std::string("pq_basis_create: copy failed: ") + hipGetErrorString(e));
     }
     for (int c = 0; c < ERI_NCLASS; ++c) {
@@ -317,6 +376,14 @@ extern "C" int pq_basis_create(int nshell, const int* l, const int* nprim, const
     }
     *out = b;
     return 0;
+}
+
+extern "C" int pq_basis_create(int nshell, const int* l, const int* nprim, const int* prim_off, const double* center,
+                               const double* exps, const double* cc, int natom, const double* atom_xyz,
+                               const double* atom_z, pq_basis** out)
+{
+    return pq_basis_create_batch(1, nshell, l, nprim, prim_off, center, exps, cc, nullptr, natom, atom_xyz, atom_z,
+                                 nullptr, out);
 }
 
 extern "C" void pq_basis_destroy(pq_basis* b)
@@ -328,6 +395,8 @@ extern "C" void pq_basis_destroy(pq_basis* b)
     }
     b->d_pairs.release(); b->d_prims.release(); b->d_atoms.release();
     b->d_cart_off.release(); b->d_bound.release(); b->d_tmp_list.release(); b->d_spher.release();
+    b->d_mols.release();
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->ev_jk[0]) {
         for (int i = 0; i < 2 * ERI_NCLASS; ++i)
             for (int k = 0; k < 2; ++k) (void)hipEventDestroy(b->ev_cls[i][k]);
@@ -359,6 +428,22 @@ extern "C" int pq_basis_info(const pq_basis* b, long long* info)
     info[1] = b->nshell;
     info[2] = b->npair;
     info[3] = b->nprimpair;
+    info[4] = b->nmol;
+    info[5] = b->mat_total;
+    info[6] = b->nmax;
+    return 0;
+}
+
+// Per molecule: base (packed offset), n (Cartesian functions), atom range.
+extern "C" int pq_basis_molecules(const pq_basis* b, long long* base, int* n, int* atom_lo, int* atom_hi)
+{
+    if (!b) return fail("pq_basis_molecules: null basis");
+    for (int m = 0; m < b->nmol; ++m) {
+        if (base) base[m] = b->mols[m].base;
+        if (n) n[m] = b->mols[m].n;
+        if (atom_lo) atom_lo[m] = b->mols[m].atom_lo;
+        if (atom_hi) atom_hi[m] = b->mols[m].atom_hi;
+    }
     return 0;
 }
 
@@ -382,10 +467,9 @@ extern "C" int pq_one_electron(pq_basis* b, double* S, double* T, double* V, voi
         a.prims = (const PrimPair<double>*)b->d_prims.p;
         a.atoms = (const double*)b->d_atoms.p;
         a.cart_off = (const int*)b->d_cart_off.p;
+        a.mols = (const MolInfo*)b->d_mols.p;
         a.S = S; a.T = T; a.V = V;
         a.npair = offs[c + 1] - offs[c];
-        a.natom = b->natom;
-        a.ncart = b->ncart;
         if (a.npair > 0) onee_run(c, a, st);
     }
     HIPCHK(hipGetLastError());
@@ -460,11 +544,19 @@ extern "C" int pq_schwarz(pq_basis* b, double* pair_bound_host, void* stream)
 // Quartets with bound >= thr32 (or all if thr32 <= 0) go to the FP64 list.
 static void build_lists(pq_basis* b, double thr, double thr32)
 {
+    // per pair class: pair ids sorted by (molecule, descending bound), plus
+    // the start of every molecule's segment
     std::vector<std::vector<int>> sorted(6);
+    std::vector<std::vector<int>> seg(6);
     for (int X = 0; X < 6; ++X) {
         sorted[X] = b->class_pairs[X];
-        std::stable_sort(sorted[X].begin(), sorted[X].end(),
-                         [&](int u, int v) { return b->bound[u] > b->bound[v]; });
+        std::stable_sort(sorted[X].begin(), sorted[X].end(), [&](int u, int v) {
+            if (b->pairs[u].mol != b->pairs[v].mol) return b->pairs[u].mol < b->pairs[v].mol;
+            return b->bound[u] > b->bound[v];
+        });
+        seg[X].assign(b->nmol + 1, 0);
+        for (int id : sorted[X]) seg[X][b->pairs[id].mol + 1]++;
+        for (int m = 0; m < b->nmol; ++m) seg[X][m + 1] += seg[X][m];
     }
     for (int c = 0; c < ERI_NCLASS; ++c) {
         b->store64[c].hq.clear(); b->store32[c].hq.clear();
@@ -478,9 +570,11 @@ static void build_lists(pq_basis* b, double thr, double thr32)
         EriClassStore& s32 = b->store32[c];
         for (size_t i = 0; i < bl.size(); ++i) {
             const int P = bl[i];
+            const int m = b->pairs[P].mol;
             const double qb = b->bound[P];
-            const size_t j0 = (X == Y) ? i : 0;
-            for (size_t j = j0; j < kl.size(); ++j) {
+            const size_t j0 = (X == Y) ? i : (size_t)seg[Y][m];
+            const size_t j1 = (size_t)seg[Y][m + 1];
+            for (size_t j = j0; j < j1; ++j) {
                 const int Q = kl[j];
                 const double est = qb * b->bound[Q];
                 const bool diag = (X == Y && j == i);
@@ -496,6 +590,34 @@ static void build_lists(pq_basis* b, double thr, double thr32)
             }
         }
     }
+}
+
+// J/K work items: runs of equal bra pair, cut into pieces of at most jk_run
+// quartets so that long runs spread over several waves.  Automatic length:
+// shrink for classes with few bra pairs so that every class still launches
+// ~2048 waves (8 per CU).
+static int make_items(EriClassStore& s)
+{
+    int run = s.jk_run;
+    if (run <= 0) {
+        run = s.nq / 2048;
+        run = run < 64 ? 64 : (run > 1024 ? 1024 : run);
+    }
+    s.hitems.clear();
+    int start = 0;
+    for (int i = 1; i <= s.nq; ++i) {
+        if (i == s.nq || s.hq[i].x != s.hq[start].x || i - start == run) {
+            s.hitems.push_back(make_int2(start, i));
+            start = i;
+        }
+    }
+    s.nitems = (int)s.hitems.size();
+    return s.items.ensure(sizeof(int2) * std::max(1, s.nitems));
+}
+
+static inline int jk_grid_of(const EriClassStore& s)
+{
+    return (s.jk_grid > 0 && s.jk_grid < s.nitems) ? s.jk_grid : s.nitems;
 }
 
 extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold, void* stream)
@@ -514,22 +636,7 @@ extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold,
             const size_t vb = (size_t)s.nq * s.ncomp * (w ? sizeof(float) : sizeof(double));
             if (s.q.ensure(sizeof(int2) * s.nq) || s.vals.ensure(vb)) return -1;
             HIPCHK(hipMemcpyAsync(s.q.p, s.hq.data(), sizeof(int2) * s.nq, hipMemcpyHostToDevice, st));
-            // J/K work items: runs of equal bra pair, cut into pieces of at most
-            // JK_RUN quartets so that long runs spread over several waves; the
-            // piece length shrinks for classes with few bra pairs so that every
-            // class still launches ~2048 waves (8 per CU)
-            int JK_RUN = s.nq / 2048;
-            JK_RUN = JK_RUN < 64 ? 64 : (JK_RUN > 1024 ? 1024 : JK_RUN);
-            s.hitems.clear();
-            int start = 0;
-            for (int i = 1; i <= s.nq; ++i) {
-                if (i == s.nq || s.hq[i].x != s.hq[start].x || i - start == JK_RUN) {
-                    s.hitems.push_back(make_int2(start, i));
-                    start = i;
-                }
-            }
-            s.nitems = (int)s.hitems.size();
-            if (s.items.ensure(sizeof(int2) * s.nitems)) return -1;
+            if (make_items(s)) return -1;
             HIPCHK(hipMemcpyAsync(s.items.p, s.hitems.data(), sizeof(int2) * s.nitems, hipMemcpyHostToDevice, st));
         }
     }
@@ -638,50 +745,99 @@ extern "C" int pq_eri_stats(const pq_basis* b, long long* stats, int nstats)
 
 extern "C" int pq_eri_nclass(void) { return ERI_NCLASS; }
 
-extern "C" int pq_jk(pq_basis* b, const double* Dj, const double* Dk, double* J, double* K, void* stream)
+static int jk_launch_all(pq_basis* b, const double* Dj, const double* Dk, double* J, double* K, hipStream_t st)
 {
-    if (!b || !Dj || !Dk || !J || !K) return fail("pq_jk: null argument");
-    hipStream_t st = (hipStream_t)stream;
-    const int n = b->ncart;
-    if (b->timing) {
-        if (ensure_events(b)) return -1;
-        HIPCHK(hipEventRecord(b->ev_jk[0], st));
-    }
-    HIPCHK(hipMemsetAsync(J, 0, sizeof(double) * (size_t)n * n, st));
-    HIPCHK(hipMemsetAsync(K, 0, sizeof(double) * (size_t)n * n, st));
+    const long long tot = b->mat_total;
+    long long blocks = (tot + 255) / 256;
+    blocks = blocks > 4096 ? 4096 : (blocks < 1 ? 1 : blocks);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, st, J, tot, K);
+    const MolInfo* dm = (const MolInfo*)b->d_mols.p;
     for (int c = 0; c < ERI_NCLASS; ++c) {
         const int nbra = ncart_l(eri_class_table[c][0]) + ncart_l(eri_class_table[c][1]);
-        const bool use_runs = sizeof(double) * (size_t)nbra * n <= 65536;
+        const bool use_runs = sizeof(double) * (size_t)nbra * b->nmax <= 65536;
         for (int w = 0; w < 2; ++w) {
             EriClassStore& s = w ? b->store32[c] : b->store64[c];
             if (!s.nq) continue;
             if (use_runs) {
                 if (w) {
                     JkRunArgs<float> a{(const int2*)s.q.p, (const int2*)s.items.p, (const PairHdr*)b->d_pairs.p,
-                                       (const int*)b->d_cart_off.p, (const float*)s.vals.p, Dj, Dk, J, K, s.nq, n,
-                                       s.nitems};
+                                       (const int*)b->d_cart_off.p, dm, (const float*)s.vals.p, Dj, Dk, J, K, s.nq,
+                                       b->nmax, s.nitems, jk_grid_of(s)};
                     jkrun_launch<float>(c, a, st);
                 } else {
                     JkRunArgs<double> a{(const int2*)s.q.p, (const int2*)s.items.p, (const PairHdr*)b->d_pairs.p,
-                                        (const int*)b->d_cart_off.p, (const double*)s.vals.p, Dj, Dk, J, K, s.nq, n,
-                                        s.nitems};
+                                        (const int*)b->d_cart_off.p, dm, (const double*)s.vals.p, Dj, Dk, J, K, s.nq,
+                                        b->nmax, s.nitems, jk_grid_of(s)};
                     jkrun_launch<double>(c, a, st);
                 }
                 continue;
             }
             if (w) {
-                JkArgs<float> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p,
-                                (const float*)s.vals.p, Dj, Dk, J, K, s.nq, n};
+                JkArgs<float> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p, dm,
+                                (const float*)s.vals.p, Dj, Dk, J, K, s.nq, b->nmax};
                 jk_launch<float>(c, a, st);
             } else {
-                JkArgs<double> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p,
-                                 (const double*)s.vals.p, Dj, Dk, J, K, s.nq, n};
+                JkArgs<double> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p, dm,
+                                 (const double*)s.vals.p, Dj, Dk, J, K, s.nq, b->nmax};
                 jk_launch<double>(c, a, st);
             }
         }
     }
-    hipLaunchKernelGGL(jk_finish_kernel, dim3((n + 127) / 128, n), dim3(128), 0, st, J, K, n);
+    int gx = (int)((b->nmax * (long long)b->nmax + 255) / 256);
+    gx = gx < 1 ? 1 : (gx > 64 ? 64 : gx);
+    hipLaunchKernelGGL(jk_finish_kernel, dim3(gx, b->nmol), dim3(256), 0, st, J, K, dm);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// Tunables of one class's J/K digestion: work-item length (run_len, 0 =
+// automatic) and blocks launched (grid, 0 = one per item; smaller = persistent).
+extern "C" int pq_jk_set_params(pq_basis* b, int cls, int run_len, int grid, void* stream)
+{
+    if (!b || cls < 0 || cls >= ERI_NCLASS) return fail("pq_jk_set_params: bad class");
+    hipStream_t st = (hipStream_t)stream;
+    for (int w = 0; w < 2; ++w) {
+        EriClassStore& s = w ? b->store32[cls] : b->store64[cls];
+        s.jk_run = run_len;
+        s.jk_grid = grid;
+        if (!s.nq) continue;
+        if (make_items(s)) return -1;
+        HIPCHK(hipMemcpyAsync(s.items.p, s.hitems.data(), sizeof(int2) * s.nitems, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// Digest one class into J', K' (no zero fill, no symmetrisation): timing unit
+// for the autotuner.  Returns the number of work items in *nitems.
+extern "C" int pq_jk_class(pq_basis* b, int cls, const double* Dj, const double* Dk, double* J, double* K,
+                           int* nitems, void* stream)
+{
+    if (!b || cls < 0 || cls >= ERI_NCLASS || !Dj || !Dk || !J || !K) return fail("pq_jk_class: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    const MolInfo* dm = (const MolInfo*)b->d_mols.p;
+    const int nbra = ncart_l(eri_class_table[cls][0]) + ncart_l(eri_class_table[cls][1]);
+    if (sizeof(double) * (size_t)nbra * b->nmax > 65536) return fail("pq_jk_class: LDS rows too large for this basis");
+    EriClassStore& s = b->store64[cls];
+    if (nitems) *nitems = s.nitems;
+    if (!s.nq) return 0;
+    JkRunArgs<double> a{(const int2*)s.q.p, (const int2*)s.items.p, (const PairHdr*)b->d_pairs.p,
+                        (const int*)b->d_cart_off.p, dm, (const double*)s.vals.p, Dj, Dk, J, K, s.nq, b->nmax,
+                        s.nitems, jk_grid_of(s)};
+    jkrun_launch<double>(cls, a, st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int pq_jk(pq_basis* b, const double* Dj, const double* Dk, double* J, double* K, void* stream)
+{
+    if (!b || !Dj || !Dk || !J || !K) return fail("pq_jk: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (b->timing) {
+        if (ensure_events(b)) return -1;
+        HIPCHK(hipEventRecord(b->ev_jk[0], st));
+    }
+    if (jk_launch_all(b, Dj, Dk, J, K, st)) return -1;
     if (b->timing) {
         HIPCHK(hipEventRecord(b->ev_jk[1], st));
         b->ev_jk_used = 1;
@@ -689,9 +845,40 @@ extern "C" int pq_jk(pq_basis* b, const double* Dj, const double* Dk, double* J,
     return 0;
 }
 
+// J/K captured once into a HIP graph (kernel nodes only) for fixed device
+// buffers, then replayed: the per-iteration Fock build of batched SCF.
+extern "C" int pq_jk_graph_create(pq_basis* b, const double* Dj, const double* Dk, double* J, double* K)
+{
+    if (!b || !Dj || !Dk || !J || !K) return fail("pq_jk_graph_create: null argument");
+    if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+    hipStream_t cs;
+    HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    const int rc = jk_launch_all(b, Dj, Dk, J, K, cs);
+    hipError_t e = hipStreamEndCapture(cs, &g);
+    if (rc || e != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        return fail(std::string("pq_jk_graph_create: capture failed: ") + hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) return fail(std::string("pq_jk_graph_create: instantiate: ") + hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int pq_jk_graph_launch(pq_basis* b, void* stream)
+{
+    if (!b || !b->graph_exec) return fail("pq_jk_graph_launch: no graph");
+    HIPCHK(hipGraphLaunch(b->graph_exec, (hipStream_t)stream));
+    return 0;
+}
+
 extern "C" int pq_eri_dense(pq_basis* b, double* out, void* stream)
 {
     if (!b || !out) return fail("pq_eri_dense: null argument");
+    if (b->nmol != 1) return fail("pq_eri_dense: single-molecule basis only");
     hipStream_t st = (hipStream_t)stream;
     const size_t n = b->ncart;
     HIPCHK(hipMemsetAsync(out, 0, sizeof(double) * n * n * n * n, st));
@@ -730,6 +917,8 @@ extern "C" int pq_eri_blocks(pq_basis* b, int n, const int* shells, double* out_
             if (s[k] < 0 || s[k] >= b->nshell) return fail("pq_eri_blocks: shell index out of range");
         int P = b->pair_index[(size_t)s[0] * b->nshell + s[1]];
         int Q = b->pair_index[(size_t)s[2] * b->nshell + s[3]];
+        if (P < 0 || Q < 0 || b->pairs[P].mol != b->pairs[Q].mol)
+            return fail("pq_eri_blocks: shells of one quartet must belong to one molecule");
         int X = b->pairs[P].pcls, Y = b->pairs[Q].pcls;
         int swapq = 0;
         if (Y > X) { std::swap(P, Q); std::swap(X, Y); swapq = 1; }
@@ -823,6 +1012,7 @@ static int eri_dense_slab(pq_basis* b, int l0, int lc, double* out, hipStream_t 
 extern "C" int pq_eri_dense_slab(pq_basis* b, int l0, int lc, double* out, void* stream)
 {
     if (!b || !out) return fail("pq_eri_dense_slab: null argument");
+    if (b->nmol != 1) return fail("pq_eri_dense_slab: single-molecule basis only");
     if (l0 < 0 || lc <= 0 || l0 + lc > b->ncart) return fail("pq_eri_dense_slab: slab out of range");
     return eri_dense_slab(b, l0, lc, out, (hipStream_t)stream);
 }
@@ -844,6 +1034,7 @@ extern "C" int pq_mp2_ovov(pq_basis* b, int o1, int v1, const double* Co1, const
 {
     if (!b || !Co1 || !Cv1 || !Co2 || !Cv2 || !ovov) return fail("pq_mp2_ovov: null argument");
     if (o1 <= 0 || v1 <= 0 || o2 <= 0 || v2 <= 0) return fail("pq_mp2_ovov: empty orbital space");
+    if (b->nmol != 1) return fail("pq_mp2_ovov: single-molecule basis only");
     hipStream_t st = (hipStream_t)stream;
     const long long n = b->ncart;
     if (max_slab_bytes <= 0) max_slab_bytes = 4LL << 30;

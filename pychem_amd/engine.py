@@ -40,9 +40,9 @@ class Engine:
                                 t["natom"], dptr(self._keepd[3]), dptr(self._keepd[4]), ctypes.byref(h)),
               "pq_basis_create")
         self.h = h
-        info = np.zeros(4, dtype=np.int64)
+        info = np.zeros(7, dtype=np.int64)
         check(L.pq_basis_info(h, info.ctypes.data_as(_lib.P_ll)), "pq_basis_info")
-        self.ncart, self.nshell, self.npair, self.nprimpair = (int(x) for x in info)
+        self.ncart, self.nshell, self.npair, self.nprimpair = (int(x) for x in info[:4])
         self.nao = t["nao"]
         self.cartesian = self.nao == self.ncart
         self.T = None if self.cartesian else torch.tensor(mol.cart_to_final(), device=self.device)
