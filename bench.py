@@ -84,6 +84,31 @@ def mp2_measure(mol, r):
             "nocc": no, "nvir": v}
 
 
+def batch_measure(count, steps):
+    """512 molecules of uneven size (H2O, NH3, CH4, C2H6, CH3OH, C3H8 in STO-3G
+    and 6-31G*, jitter 0.05 A, default_rng(4)) in one batched RHF; the J/K
+    launch shapes come from the autotuner (tuned on an untimed pass)."""
+    import torch
+    from pychem_amd.batch import config4_molecules, rhf_batch
+    mols = config4_molecules(count, seed=4)
+    warm = rhf_batch(mols, conv=1e-8, maxiter=80, tune="search")
+    table = warm["tuned"]
+    runs = []
+    for _ in range(max(1, steps)):
+        runs.append(rhf_batch(config4_molecules(count, seed=4), conv=1e-8, maxiter=80, tune=table))
+    untuned = rhf_batch(config4_molecules(count, seed=4), conv=1e-8, maxiter=80)
+    r = runs[-1]
+    tot = float(np.median([x["total_s"] for x in runs]))
+    lat = r["latency_s"]
+    nbf = [m.NOrbitals for m in mols]
+    return {"molecules": count, "nbf_min": int(min(nbf)), "nbf_max": int(max(nbf)),
+            "molecules_per_s": count / tot, "wall_s": tot, "setup_s": r["setup_s"],
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+            "iterations_max": int(r["iterations"].max()), "all_converged": bool(r["converged"].all()),
+            "untuned_wall_s": untuned["total_s"],
+            "tuned_jk": {str(k): {kk: v[kk] for kk in ("run_len", "grid", "ms", "default_ms")} for k, v in table.items()}}
+
+
 def cpu_baseline(name, threads):
     """The oracle's full RHF (Schwarz, dense screened ERIs, einsum J/K, DIIS)
     on the host cores for the same molecule."""
@@ -104,6 +129,8 @@ def main():
     ap.add_argument("--workload", default="c10h22")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-size", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -192,6 +219,8 @@ def main():
                   "eri_pass_ms": float(np.sum(np.clip(mixed[-1][0]["timings"]["eri_class_ms"], 0, None))),
                   "jk_ms": float(np.mean(mixed[-1][0]["timings"]["jk_ms"]))},
     }
+    if not args.no_batch:
+        out["batch"] = batch_measure(args.batch_size, args.steps)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         cb = cpu_baseline(args.workload, threads)

@@ -136,7 +136,8 @@ def _sad_padded(eng, mols):
     return D.to(eng.device)
 
 
-def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_graph=True, diis_size=8):
+def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_graph=True, diis_size=8,
+              tune=None):
     """Closed-shell RHF for every molecule in ``mols`` at once.
 
     Returns dict: energies [B], iterations [B], converged [B], per-molecule
@@ -174,6 +175,17 @@ def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_grap
         e, C = torch.linalg.eigh(M)
         C = X @ C
         D = (C * occ[:, None, :]) @ C.transpose(1, 2)
+    if tune is not None:
+        from . import autotune
+        eng.Dj = eng.to_packed(2.0 * D).contiguous()
+        eng.Dk = eng.to_packed(D).contiguous()
+        eng.J = torch.zeros_like(eng.Dj)
+        eng.K = torch.zeros_like(eng.Dj)
+        if tune == "search":
+            eng.tuned = autotune.tune_jk(eng, eng.Dj, eng.Dk, eng.J, eng.K)
+        else:
+            autotune.apply(eng, tune)
+            eng.tuned = tune
     if use_graph:
         eng.jk_buffers()
     else:
@@ -239,7 +251,8 @@ def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_grap
     total = time.perf_counter() - t_start
     latency[np.isnan(latency)] = total
     out = dict(energies=E.cpu().numpy(), iterations=iters.cpu().numpy(), converged=done.cpu().numpy(),
-               latency_s=latency, total_s=total, setup_s=t_setup, nmol=B, stats=eng.stats())
+               latency_s=latency, total_s=total, setup_s=t_setup, nmol=B, stats=eng.stats(),
+               tuned=getattr(eng, "tuned", None))
     eng.close()
     return out
 
