@@ -43,12 +43,27 @@ int pq_basis_create(int nshell, const int* l, const int* nprim, const int* prim_
                     const double* center, const double* exps, const double* cc,
                     int natom, const double* atom_xyz, const double* atom_z,
                     pq_basis** out);
+/* Many molecules in one handle (no reference counterpart: the reference
+ * builds one Molecule at a time, Util/structures.py:319-806).  shell_mol[nshell]
+ * and atom_mol[natom] give each shell's and atom's molecule (0..nmol-1,
+ * non-decreasing).  Shell pairs, quartets and the one-electron nuclear sum
+ * stay inside one molecule; matrices are packed per molecule, molecule m's
+ * n_m x n_m block starting at element base_m (pq_basis_molecules).  NULL
+ * shell_mol/atom_mol with nmol = 1 is pq_basis_create. */
+int pq_basis_create_batch(int nmol, int nshell, const int* l, const int* nprim, const int* prim_off,
+                          const double* center, const double* exps, const double* cc, const int* shell_mol,
+                          int natom, const double* atom_xyz, const double* atom_z, const int* atom_mol,
+                          pq_basis** out);
+/* Per molecule m < nmol: packed base offset, Cartesian size n, atom range
+ * [atom_lo, atom_hi).  Any output may be NULL. */
+int pq_basis_molecules(const pq_basis* b, long long* base, int* n, int* atom_lo, int* atom_hi);
 void pq_basis_destroy(pq_basis* b);
 /* Mark spherical (5d) shells, spher[nshell] in {0,1} (Util/structures.py:844-849).
  * Only affects Schwarz bounds, which the reference takes in the final basis;
  * all matrices stay Cartesian and the caller applies cart->spherical. */
 int pq_basis_set_spherical(pq_basis* b, const int* spher);
-/* info[0..3] = ncart, nshell, npair, nprimpair. */
+/* info[0..6] = ncart, nshell, npair, nprimpair, nmol, packed matrix
+ * elements (sum of n_m^2), largest n_m. */
 int pq_basis_info(const pq_basis* b, long long* info);
 
 /* Overlap, kinetic and nuclear-attraction matrices (Cartesian, normalised).
@@ -149,6 +164,24 @@ int pq_mp2_ovov(pq_basis* b, int o1, int v1, const double* Co1, const double* Cv
  * (Methods/mp2.py:77-94).  Orbital energies are device arrays. */
 int pq_mp2_energy(const double* ovov_dev, const double* eo1, const double* ev1, const double* eo2,
                   const double* ev2, int o1, int v1, int o2, int v2, int mode, double* out_host, void* stream);
+
+/* Learned SCF guess: kernels of a small FP32 multilayer perceptron (no
+ * reference counterpart: the reference has only the core guess,
+ * Methods/hartree_fock.py:55-57).  X [B x I], W [I x O], b [O], Z/A [B x O];
+ * act 0 = identity, 1 = tanh.
+ * fwd: Z = X W + b, A = act(Z) (A may be NULL).
+ * bwd: dZ = dA act'(Z) (A = act(Z)), dX = dZ W^T (dX may be NULL),
+ *      dW = X^T dZ, db = column sums of dZ.
+ * mse: loss += sum mask (Y-T)^2 * inv_norm (device scalar), dY = 2 mask (Y-T) inv_norm.
+ * adam: one bias-corrected Adam step at step number `step` (>= 1). */
+int pq_mlp_linear_fwd(const float* X, const float* W, const float* b, float* Z, float* A, int B, int I, int O,
+                      int act, void* stream);
+int pq_mlp_linear_bwd(const float* X, const float* W, const float* A, const float* dA, float* dZ, float* dX,
+                      float* dW, float* db, int B, int I, int O, int act, void* stream);
+int pq_mlp_mse(const float* Y, const float* T, const float* mask, float* dY, float* loss_dev, long long n,
+               float inv_norm, void* stream);
+int pq_mlp_adam(float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2, float eps,
+                int step, void* stream);
 
 #ifdef __cplusplus
 }

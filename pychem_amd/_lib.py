@@ -53,6 +53,13 @@ SIGNATURES = {
                             c_int, c_void_p]),
     "pq_mp2_ovov": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
                             c_void_p, c_longlong, c_void_p]),
+    "pq_mlp_linear_fwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
+                                  c_void_p]),
+    "pq_mlp_linear_bwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_int, c_int, c_int, c_int, c_void_p]),
+    "pq_mlp_mse": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_longlong, ctypes.c_float, c_void_p]),
+    "pq_mlp_adam": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_longlong, ctypes.c_float, ctypes.c_float,
+                            ctypes.c_float, ctypes.c_float, c_int, c_void_p]),
     "pq_mp2_energy": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
                               c_int, P_double, c_void_p]),
 }

@@ -137,7 +137,7 @@ def _sad_padded(eng, mols):
 
 
 def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_graph=True, diis_size=8,
-              tune=None):
+              tune=None, return_density=False):
     """Closed-shell RHF for every molecule in ``mols`` at once.
 
     Returns dict: energies [B], iterations [B], converged [B], per-molecule
@@ -253,6 +253,9 @@ def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_grap
     out = dict(energies=E.cpu().numpy(), iterations=iters.cpu().numpy(), converged=done.cpu().numpy(),
                latency_s=latency, total_s=total, setup_s=t_setup, nmol=B, stats=eng.stats(),
                tuned=getattr(eng, "tuned", None))
+    if return_density:
+        Dh = D.cpu().numpy()
+        out["densities"] = [Dh[m, :eng.n[m], :eng.n[m]].copy() for m in range(B)]
     eng.close()
     return out
 

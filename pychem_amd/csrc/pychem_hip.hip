@@ -24,6 +24,7 @@
 #include "eri_common.h"
 #include "eri_dispatch.h"
 #include "mp2.h"
+#include "mlp.h"
 
 
 __global__ void boys_test_kernel(const double* T, int n, int mmax, double* out)
@@ -1091,5 +1092,43 @@ extern "C" int pq_mp2_energy(const double* ovov, const double* eo1, const double
     HIPCHK(hipMemcpyAsync(out_host, acc.p, sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     acc.release();
+    return 0;
+}
+
+// --------------------------------------------------------------------------
+// Learned-guess MLP (FP32)
+// --------------------------------------------------------------------------
+extern "C" int pq_mlp_linear_fwd(const float* X, const float* W, const float* b, float* Z, float* A, int B, int I,
+                                 int O, int act, void* stream)
+{
+    if (!X || !W || !Z) return fail("pq_mlp_linear_fwd: null argument");
+    if (act < 0 || act > 1) return fail("pq_mlp_linear_fwd: act must be 0 or 1");
+    if (mlp_linear_fwd(X, W, b, Z, A, B, I, O, act, (hipStream_t)stream)) return fail("pq_mlp_linear_fwd: launch failed");
+    return 0;
+}
+
+extern "C" int pq_mlp_linear_bwd(const float* X, const float* W, const float* A, const float* dA, float* dZ,
+                                 float* dX, float* dW, float* db, int B, int I, int O, int act, void* stream)
+{
+    if (!X || !W || !A || !dA || !dZ || !dW || !db) return fail("pq_mlp_linear_bwd: null argument");
+    if (act < 0 || act > 1) return fail("pq_mlp_linear_bwd: act must be 0 or 1");
+    if (mlp_linear_bwd(X, W, A, dA, dZ, dX, dW, db, B, I, O, act, (hipStream_t)stream))
+        return fail("pq_mlp_linear_bwd: launch failed");
+    return 0;
+}
+
+extern "C" int pq_mlp_mse(const float* Y, const float* T, const float* mask, float* dY, float* loss_dev, long long n,
+                          float inv_norm, void* stream)
+{
+    if (!Y || !T || !dY || !loss_dev) return fail("pq_mlp_mse: null argument");
+    if (mlp_mse(Y, T, mask, dY, loss_dev, n, inv_norm, (hipStream_t)stream)) return fail("pq_mlp_mse: launch failed");
+    return 0;
+}
+
+extern "C" int pq_mlp_adam(float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2,
+                           float eps, int step, void* stream)
+{
+    if (!p || !g || !m || !v) return fail("pq_mlp_adam: null argument");
+    if (mlp_adam(p, g, m, v, n, lr, b1, b2, eps, step, (hipStream_t)stream)) return fail("pq_mlp_adam: launch failed");
     return 0;
 }

@@ -1,0 +1,65 @@
+"""Learned-guess host logic on the CPU: the Cartesian rotation law of density
+blocks (checked with two oracle SCFs on rotated geometries), invariance of the
+descriptors, and the NumPy restatement of the MLP gradients."""
+import math
+
+import numpy as np
+
+from pychem_amd import geometry, mlp_guess, structures
+from oracle import scf as oscf
+
+
+def _rot(a, b, c):
+    Rz = np.array([[math.cos(a), -math.sin(a), 0], [math.sin(a), math.cos(a), 0], [0, 0, 1]])
+    Ry = np.array([[math.cos(b), 0, math.sin(b)], [0, 1, 0], [-math.sin(b), 0, math.cos(b)]])
+    Rx = np.array([[1, 0, 0], [0, math.cos(c), -math.sin(c)], [0, math.sin(c), math.cos(c)]])
+    return Rz @ Ry @ Rx
+
+
+def _mol(rows):
+    return structures.Molecule(rows, basis="6-31G*", cartesian_l=[2])
+
+
+def test_density_rotation_law():
+    """D(rotated geometry) = M^-T D M^-1 with M = block-diag atom_rotation(R)."""
+    rows = geometry.small_molecule("CH3OH", 0.05, 3)
+    R = _rot(0.4, -0.9, 1.3)
+    rows_r = [[r[0], r[1]] + list(R @ np.array(r[2:], dtype=float)) for r in rows]
+    m, mr = _mol(rows), _mol(rows_r)
+    D = oscf.run_rhf(m, conv=1e-12, diis=oscf.PULAY)["Da"]
+    Dr = oscf.run_rhf(mr, conv=1e-12, diis=oscf.PULAY)["Da"]
+    idx, ls = mlp_guess._atom_functions(m)
+    n = m.NOrbitals
+    M = np.zeros((n, n))
+    for a in range(m.NAtom):
+        M[np.ix_(idx[a], idx[a])] = mlp_guess.atom_rotation(ls[a], R)
+    pred = np.linalg.solve(M.T, D) @ np.linalg.inv(M)
+    assert np.abs(pred - Dr).max() < 1e-7
+
+
+def test_local_blocks_and_descriptors_are_rotation_invariant():
+    rows = geometry.alcohol(2, 0.05, 9)
+    R = _rot(-1.1, 0.3, 2.0)
+    rows_r = [[r[0], r[1]] + list(R @ np.array(r[2:], dtype=float)) for r in rows]
+    m, mr = _mol(rows), _mol(rows_r)
+    D = oscf.run_rhf(m, conv=1e-12, diis=oscf.PULAY)["Da"]
+    Dr = oscf.run_rhf(mr, conv=1e-12, diis=oscf.PULAY)["Da"]
+    s1, s2 = mlp_guess.pair_samples(m), mlp_guess.pair_samples(mr)
+    assert len(s1) == len(s2)
+    for (k1, a1, b1, f1, R1), (k2, a2, b2, f2, R2) in zip(s1, s2):
+        assert k1 == k2 and a1 == a2 and b1 == b2
+        assert np.abs(f1 - f2).max() < 1e-9
+        L1 = mlp_guess.local_block(D, m, a1, b1, R1)
+        L2 = mlp_guess.local_block(Dr, mr, a2, b2, R2)
+        assert np.abs(L1 - L2).max() < 1e-7
+
+
+def test_purify_idempotent():
+    m = _mol(geometry.water())
+    r = oscf.run_rhf(m, conv=1e-12, diis=oscf.PULAY)
+    S = r["S"]
+    D = r["Da"] * 1.07 + 1e-3
+    D = 0.5 * (D + D.T)
+    P = mlp_guess.purify(D, S, m.NAlphaElectrons, rounds=6)
+    assert abs(np.trace(P @ S) - m.NAlphaElectrons) < 1e-6
+    assert np.abs(P @ S @ P - P).max() < 1e-6
