@@ -31,6 +31,7 @@ SIGNATURES = {
     "pq_jk_class": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, P_int, c_void_p]),
     "pq_jk_graph_create": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pq_jk_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "pq_jk_set_streams": (c_int, [c_void_p, c_int]),
     "pq_basis_set_spherical": (c_int, [c_void_p, P_int]),
     "pq_basis_info": (c_int, [c_void_p, P_ll]),
     "pq_one_electron": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

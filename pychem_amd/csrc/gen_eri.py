@@ -346,6 +346,25 @@ def _write_if_changed(path, text):
         fh.write(text)
 
 
+def jk_tile(la, lb, lc, ld, budget=72):
+    """Bra tile (TA, TB) of the tiled J/K kernel: the fewest tiles whose
+    per-lane live state, 2 (TA + TB)(N4 + 1) + 2 TA TB doubles, fits the
+    budget (ties: the smaller state).  Returns (TA, TB, number of tiles)."""
+    nc = lambda l: (l + 1) * (l + 2) // 2
+    n1, n2, n4 = nc(la), nc(lb), nc(ld)
+    best = None
+    for ta in [t for t in range(1, n1 + 1) if n1 % t == 0]:
+        for tb in [t for t in range(1, n2 + 1) if n2 % t == 0]:
+            cost = 2 * (ta + tb) * (n4 + 1) + 2 * ta * tb
+            if cost > budget and (ta, tb) != (1, 1):
+                continue
+            key = ((n1 // ta) * (n2 // tb), cost)
+            if best is None or key < best[0]:
+                best = (key, ta, tb)
+    _, ta, tb = best
+    return ta, tb, (n1 // ta) * (n2 // tb)
+
+
 def main():
     here = os.path.dirname(os.path.abspath(__file__))
     gdir = os.path.join(here, "gen")
@@ -366,9 +385,10 @@ def main():
             src.append("{ hipLaunchKernelGGL(eri_k%s<%s>, dim3((a.nq + ERI_BLOCK - 1) / ERI_BLOCK), dim3(ERI_BLOCK), 0, st, a); }" % (tag, rt))
             jsrc.append("void jk_run_%s_%s(const JkArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
             jsrc.append("{ hipLaunchKernelGGL((jk_kernel<%d, %d, %d, %d, %s>), dim3((a.nq + 127) / 128), dim3(128), 0, st, a); }" % (la, lb, lc, ld, rt))
-            jsrc.append("void jkrun_run_%s_%s(const JkRunArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
-            jsrc.append("{ hipLaunchKernelGGL((jk_run_kernel<%d, %d, %d, %d, %s>), dim3(a.grid), dim3(64), "
-                        "sizeof(double) * %d * a.n, st, a); }" % (la, lb, lc, ld, rt, (la + 1) * (la + 2) // 2 + (lb + 1) * (lb + 2) // 2))
+            ta, tb, nt = jk_tile(la, lb, lc, ld)
+            jsrc.append("void jktile_run_%s_%s(const JkTileArgs<%s>& a, int block, hipStream_t st)" % (tag, rn, rt))
+            jsrc.append("{ hipLaunchKernelGGL((jk_tile_kernel<%d, %d, %d, %d, %d, %d, %s>), dim3(a.grid, %d), dim3(block), "
+                        "sizeof(double) * %d * a.nmax, st, a); }" % (la, lb, lc, ld, ta, tb, rt, nt, 2 * (ta + tb)))
             jsrc.append("void scatter_run_%s_%s(const ScatterArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
             jsrc.append("{ hipLaunchKernelGGL((scatter_kernel<%d, %d, %d, %d, %s>), dim3((a.nq + 127) / 128), dim3(128), 0, st, a); }" % (la, lb, lc, ld, rt))
         _write_if_changed(os.path.join(gdir, "cls_%s.hip" % tag), "\n".join(src) + "\n")
@@ -394,7 +414,7 @@ def main():
         for rn, rt in (("d", "double"), ("f", "float")):
             d.append("void eri_run_%s_%s(const EriArgs<%s>&, hipStream_t);" % (tag, rn, rt))
             d.append("void jk_run_%s_%s(const JkArgs<%s>&, hipStream_t);" % (tag, rn, rt))
-            d.append("void jkrun_run_%s_%s(const JkRunArgs<%s>&, hipStream_t);" % (tag, rn, rt))
+            d.append("void jktile_run_%s_%s(const JkTileArgs<%s>&, int, hipStream_t);" % (tag, rn, rt))
             d.append("void scatter_run_%s_%s(const ScatterArgs<%s>&, hipStream_t);" % (tag, rn, rt))
     d.append("void onee_run(int pcls, const OneeArgs& a, hipStream_t st);")
     d.append("// la, lb, lc, ld, components, bra pair class, ket pair class")
@@ -402,7 +422,20 @@ def main():
     for la, lb, lc, ld, ncomp, X, Y, nbody in table:
         d.append("  {%d, %d, %d, %d, %d, %d, %d}," % (la, lb, lc, ld, ncomp, X, Y))
     d.append("};")
-    for kname, targ in (("eri", "EriArgs"), ("jk", "JkArgs"), ("jkrun", "JkRunArgs"), ("scatter", "ScatterArgs")):
+    d.append("// tiled J/K: bra tile TA x TB, tiles per quartet, LDS doubles per basis function (2 (TA + TB))")
+    d.append("static const int jk_tile_table[ERI_NCLASS][4] = {")
+    for la, lb, lc, ld, ncomp, X, Y, nbody in table:
+        ta, tb, nt = jk_tile(la, lb, lc, ld)
+        d.append("  {%d, %d, %d, %d}," % (ta, tb, nt, 2 * (ta + tb)))
+    d.append("};")
+    for rn, rt in (("d", "double"), ("f", "float")):
+        d.append("static void (*const jktile_table_%s[ERI_NCLASS])(const JkTileArgs<%s>&, int, hipStream_t) = {" % (rn, rt))
+        d.append("  " + ", ".join("jktile_run_%d%d%d%d_%s" % (t[0], t[1], t[2], t[3], rn) for t in table))
+        d.append("};")
+    d.append("template <typename real> static inline void jktile_launch(int cls, const JkTileArgs<real>& a, int block, hipStream_t st)")
+    d.append("{ if (sizeof(real) == 8) jktile_table_d[cls]((const JkTileArgs<double>&)a, block, st); "
+             "else jktile_table_f[cls]((const JkTileArgs<float>&)a, block, st); }")
+    for kname, targ in (("eri", "EriArgs"), ("jk", "JkArgs"), ("scatter", "ScatterArgs")):
         for rn, rt in (("d", "double"), ("f", "float")):
             d.append("static void (*const %s_table_%s[ERI_NCLASS])(const %s<%s>&, hipStream_t) = {" % (kname, rn, targ, rt))
             d.append("  " + ", ".join("%s_run_%d%d%d%d_%s" % (kname, t[0], t[1], t[2], t[3], rn) for t in table))

@@ -116,128 +116,164 @@ __global__ void __launch_bounds__(128) jk_kernel(const JkArgs<real> a)
     }
 }
 
-// Run-based J/K: one wave per work item = a contiguous range of quartets that
+// Tiled J/K digestion.  A work item is a contiguous range of quartets that
 // share one bra pair (a,b) (the class lists are sorted by bra pair).  Every K
-// contribution of such a range lands in the rows of shells a and b, so those
-// (N1+N2) rows are accumulated in LDS and flushed to global memory once per
-// range; the bra J block is summed over the wave.  Only the ket J block is
-// written with per-quartet global atomics.
+// contribution of such a range lands in rows of shells a and b, so those rows
+// are accumulated in LDS and flushed to global memory once per item, and the
+// bra rows of Dk that every K term reads are staged in LDS beside them.  The
+// bra-side J block is summed over each wave.  The ket J block of every
+// quartet is stored, not added: jkbuf[(tile * N3 * N4 + cd) * nq + q] holds
+// 4 s sum_ab Dj_ab (ab|cd) for it, and a reduction over the bras that reach
+// each ket (jket_reduce) adds those into J' with one atomic per ket element.
+//
+// The bra components are cut into tiles of TA x TB (first x second shell
+// components); blockIdx.y selects the tile.  Each lane holds, per quartet,
+// TA*TB*N3*N4 values and 2(TA+TB)(N4+1) + 2 TA TB accumulators and density
+// elements, which is what bounds the register file: the generator picks the
+// tile per class.  The ket loop runs c outer, d inner, so the ket J sum of
+// element (c,d) is complete in a register when it is written.
+struct JkItem {
+    int start, end;   // quartet range [start, end)
+    int bra;          // o1 | o2 << 15 | (s1 == s2) << 30, offsets local to the molecule
+    int mol;
+};
+
 template <typename real>
-struct JkRunArgs {
-    const int2* quartets;
-    const int2* items;       // [start, end) per work item, one bra pair each
-    const PairHdr* pairs;
-    const int* cart_off;
+struct JkTileArgs {
+    const JkItem* items;
+    const unsigned* kets;    // per quartet: o3 | o4 << 15 | (s3 == s4) << 30 | (P == Q) << 31
     const MolInfo* mols;
     const real* vals;
     const double* Dj;
     const double* Dk;
     double* J;
     double* K;
-    int nq, n, nitems, grid;   // grid: blocks launched (<= nitems; persistent when smaller)
+    double* jkbuf;           // [ntiles * N3 * N4][nq] ket J values per quartet
+    int nq, nmax, nitems, grid;   // grid: blocks in x (<= nitems; persistent when smaller)
 };
 
-template <int LA, int LB, int LC, int LD, typename real>
-__global__ void __launch_bounds__(64) jk_run_kernel(const JkRunArgs<real> a)
+template <int LA, int LB, int LC, int LD, int TA, int TB, typename real>
+__global__ void __launch_bounds__(256) jk_tile_kernel(const JkTileArgs<real> a)
 {
     constexpr int N1 = (LA + 1) * (LA + 2) / 2, N2 = (LB + 1) * (LB + 2) / 2;
     constexpr int N3 = (LC + 1) * (LC + 2) / 2, N4 = (LD + 1) * (LD + 2) / 2;
-    extern __shared__ double krows[];   // (N1 + N2) x n
-    // persistent: the grid may be smaller than the number of work items
-    for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
-    const int2 rng = a.items[item];
-    const int lane = threadIdx.x;
-    const int2 pr0 = a.quartets[rng.x];
-    const PairHdr bh = a.pairs[pr0.x];
-    const MolInfo mi = a.mols[bh.mol];
-    const int n = mi.n;
-    const int nrow = (N1 + N2) * n;
-    for (int i = lane; i < nrow; i += 64) krows[i] = 0.0;
-    __syncthreads();
-    const int o1 = a.cart_off[bh.s1], o2 = a.cart_off[bh.s2];
-    const double sbra = (bh.s1 == bh.s2) ? 0.5 : 1.0;
-    const double* Dj = a.Dj + mi.base;
-    const double* Dk = a.Dk + mi.base;
-    double* const Jm = a.J + mi.base;
-    double* const Km = a.K + mi.base;
+    static_assert(N1 % TA == 0 && N2 % TB == 0, "tile must divide the bra components");
+    constexpr int NTB = N2 / TB, NR = TA + TB;
+    extern __shared__ double lds[];
+    const int tile = blockIdx.y;
+    const int a0 = (tile / NTB) * TA, b0 = (tile % NTB) * TB;
+    const int tid = threadIdx.x, nt = blockDim.x;
     const size_t nq = a.nq;
-    double j12[N1 * N2];
+    const real* __restrict__ vals = a.vals;
+    for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
+        const JkItem it = a.items[item];
+        const MolInfo mi = a.mols[it.mol];
+        const int n = mi.n;
+        double* drows = lds;            // NR rows of Dk: o1+a0.., then o2+b0..
+        double* krows = lds + NR * n;   // K' rows, same order
+        const int o1 = it.bra & 0x7fff, o2 = (it.bra >> 15) & 0x7fff;
+        const double sbra = ((it.bra >> 30) & 1) ? 0.5 : 1.0;
+        const double* __restrict__ Dj = a.Dj + mi.base;
+        const double* __restrict__ Dk = a.Dk + mi.base;
+        double* const Jm = a.J + mi.base;
+        double* const Km = a.K + mi.base;
+        for (int i = tid; i < NR * n; i += nt) {
+            const int r = i / n, c = i - r * n;
+            const int row = r < TA ? o1 + a0 + r : o2 + b0 + (r - TA);
+            drows[i] = Dk[(size_t)row * n + c];
+            krows[i] = 0.0;
+        }
+        double d12[TA][TB], j12[TA][TB];
 #pragma unroll
-    for (int i = 0; i < N1 * N2; ++i) j12[i] = 0.0;
-    for (int q = rng.x + lane; q < rng.y; q += 64) {
-        const int2 pr = a.quartets[q];
-        const PairHdr kh = a.pairs[pr.y];
-        const int o3 = a.cart_off[kh.s1], o4 = a.cart_off[kh.s2];
-        double sc = sbra;
-        if (kh.s1 == kh.s2) sc *= 0.5;
-        if (pr.x == pr.y) sc *= 0.5;
-        double j34[N3 * N4], k23[N2 * N3], k24[N2 * N4];
+        for (int x = 0; x < TA; ++x)
 #pragma unroll
-        for (int i = 0; i < N3 * N4; ++i) j34[i] = 0.0;
+            for (int y = 0; y < TB; ++y) {
+                d12[x][y] = Dj[(size_t)(o1 + a0 + x) * n + o2 + b0 + y];
+                j12[x][y] = 0.0;
+            }
+        __syncthreads();
+        for (int q = it.start + tid; q < it.end; q += nt) {
+            const unsigned w = a.kets[q];
+            const int o3 = w & 0x7fff, o4 = (w >> 15) & 0x7fff;
+            double sc = sbra;
+            if ((w >> 30) & 1u) sc *= 0.5;
+            if (w >> 31) sc *= 0.5;
+            double d14[TA][N4], d24[TB][N4], k14[TA][N4], k24[TB][N4];
 #pragma unroll
-        for (int i = 0; i < N2 * N3; ++i) k23[i] = 0.0;
+            for (int d = 0; d < N4; ++d) {
 #pragma unroll
-        for (int i = 0; i < N2 * N4; ++i) k24[i] = 0.0;
-#pragma unroll
-        for (int ia = 0; ia < N1; ++ia) {
-            double k13[N3], k14[N4];
-#pragma unroll
-            for (int i = 0; i < N3; ++i) k13[i] = 0.0;
-#pragma unroll
-            for (int i = 0; i < N4; ++i) k14[i] = 0.0;
-#pragma unroll
-            for (int ib = 0; ib < N2; ++ib) {
-                const double d12 = Dj[(size_t)(o1 + ia) * n + o2 + ib];
-                double jab = 0.0;
-#pragma unroll
-                for (int ic = 0; ic < N3; ++ic) {
-                    const double d13 = Dk[(size_t)(o1 + ia) * n + o3 + ic];
-                    const double d23 = Dk[(size_t)(o2 + ib) * n + o3 + ic];
-#pragma unroll
-                    for (int id = 0; id < N4; ++id) {
-                        const double v = (double)a.vals[(size_t)(((ia * N2 + ib) * N3 + ic) * N4 + id) * nq + q];
-                        jab += Dj[(size_t)(o3 + ic) * n + o4 + id] * v;
-                        j34[ic * N4 + id] += d12 * v;
-                        k13[ic] += Dk[(size_t)(o2 + ib) * n + o4 + id] * v;
-                        k23[ib * N3 + ic] += Dk[(size_t)(o1 + ia) * n + o4 + id] * v;
-                        k14[id] += d23 * v;
-                        k24[ib * N4 + id] += d13 * v;
-                    }
+                for (int x = 0; x < TA; ++x) {
+                    d14[x][d] = drows[x * n + o4 + d];
+                    k14[x][d] = 0.0;
                 }
-                j12[ia * N2 + ib] += sc * jab;
+#pragma unroll
+                for (int y = 0; y < TB; ++y) {
+                    d24[y][d] = drows[(TA + y) * n + o4 + d];
+                    k24[y][d] = 0.0;
+                }
             }
 #pragma unroll
-            for (int ic = 0; ic < N3; ++ic) atomicAdd(&krows[ia * n + o3 + ic], 2.0 * sc * k13[ic]);
+            for (int c = 0; c < N3; ++c) {
+                double d13[TA], d23[TB], k13[TA], k23[TB];
 #pragma unroll
-            for (int id = 0; id < N4; ++id) atomicAdd(&krows[ia * n + o4 + id], 2.0 * sc * k14[id]);
+                for (int x = 0; x < TA; ++x) {
+                    d13[x] = drows[x * n + o3 + c];
+                    k13[x] = 0.0;
+                }
+#pragma unroll
+                for (int y = 0; y < TB; ++y) {
+                    d23[y] = drows[(TA + y) * n + o3 + c];
+                    k23[y] = 0.0;
+                }
+#pragma unroll
+                for (int d = 0; d < N4; ++d) {
+                    const double djcd = Dj[(size_t)(o3 + c) * n + o4 + d];
+                    double jcd = 0.0;
+#pragma unroll
+                    for (int x = 0; x < TA; ++x)
+#pragma unroll
+                        for (int y = 0; y < TB; ++y) {
+                            const int comp = (((a0 + x) * N2 + b0 + y) * N3 + c) * N4 + d;
+                            const double v = sc * (double)vals[(size_t)comp * nq + q];
+                            j12[x][y] += djcd * v;
+                            jcd += d12[x][y] * v;
+                            k13[x] += d24[y][d] * v;
+                            k14[x][d] += d23[y] * v;
+                            k23[y] += d14[x][d] * v;
+                            k24[y][d] += d13[x] * v;
+                        }
+                    a.jkbuf[(size_t)(tile * N3 * N4 + c * N4 + d) * nq + q] = 4.0 * jcd;
+                }
+#pragma unroll
+                for (int x = 0; x < TA; ++x) atomicAdd(&krows[x * n + o3 + c], 2.0 * k13[x]);
+#pragma unroll
+                for (int y = 0; y < TB; ++y) atomicAdd(&krows[(TA + y) * n + o3 + c], 2.0 * k23[y]);
+            }
+#pragma unroll
+            for (int d = 0; d < N4; ++d) {
+#pragma unroll
+                for (int x = 0; x < TA; ++x) atomicAdd(&krows[x * n + o4 + d], 2.0 * k14[x][d]);
+#pragma unroll
+                for (int y = 0; y < TB; ++y) atomicAdd(&krows[(TA + y) * n + o4 + d], 2.0 * k24[y][d]);
+            }
         }
 #pragma unroll
-        for (int ic = 0; ic < N3; ++ic)
+        for (int x = 0; x < TA; ++x)
 #pragma unroll
-            for (int id = 0; id < N4; ++id) atomic_addd(&Jm[(size_t)(o3 + ic) * n + o4 + id], 4.0 * sc * j34[ic * N4 + id]);
-#pragma unroll
-        for (int ib = 0; ib < N2; ++ib) {
-#pragma unroll
-            for (int ic = 0; ic < N3; ++ic) atomicAdd(&krows[(N1 + ib) * n + o3 + ic], 2.0 * sc * k23[ib * N3 + ic]);
-#pragma unroll
-            for (int id = 0; id < N4; ++id) atomicAdd(&krows[(N1 + ib) * n + o4 + id], 2.0 * sc * k24[ib * N4 + id]);
+            for (int y = 0; y < TB; ++y) {
+                const double s = wave_sum(4.0 * j12[x][y]);
+                if ((tid & 63) == 0 && s != 0.0) atomic_addd(&Jm[(size_t)(o1 + a0 + x) * n + o2 + b0 + y], s);
+            }
+        __syncthreads();
+        for (int i = tid; i < NR * n; i += nt) {
+            const double v = krows[i];
+            if (v != 0.0) {
+                const int r = i / n, c = i - r * n;
+                const int row = r < TA ? o1 + a0 + r : o2 + b0 + (r - TA);
+                atomic_addd(&Km[(size_t)row * n + c], v);
+            }
         }
-    }
-#pragma unroll
-    for (int i = 0; i < N1 * N2; ++i) {
-        const double s = wave_sum(4.0 * j12[i]);
-        if (lane == 0) atomic_addd(&Jm[(size_t)(o1 + i / N2) * n + o2 + i % N2], s);
-    }
-    __syncthreads();
-    for (int i = lane; i < nrow; i += 64) {
-        const double v = krows[i];
-        if (v != 0.0) {
-            const int r = i / n, c = i - r * n;
-            const int row = r < N1 ? o1 + r : o2 + (r - N1);
-            atomic_addd(&Km[(size_t)row * n + c], v);
-        }
-    }
-    __syncthreads();
+        __syncthreads();
     }
 }
 

@@ -1,0 +1,51 @@
+// lists.h -- screened quartet lists built on the device, and the ket-side J
+// reduction of the tiled J/K digestion.
+//
+// Pairs of one pair class are sorted by (molecule, Schwarz bound descending).
+// For a bra pair i the kept kets are then a contiguous run of the sorted ket
+// list: all kets j of the molecule (j >= i when bra and ket class agree) with
+// bound_i * bound_j > thr, the diagonal (i|i) always included.  Quartets with
+// bound product below thr32 form a second (FP32) list; since the product falls
+// monotonically along the ket list, each bra's FP64 kets are a prefix of its
+// run and its FP32 kets the rest.  The same monotonicity in the bra index
+// makes the bras that reach a given ket j a contiguous range, which is what
+// the ket J reduction walks.
+#pragma once
+#include "jk_common.h"
+
+struct ListArgs {
+    const int* bra;              // bra pair ids sorted by (molecule, bound desc)
+    const int* ket;              // ket pair ids, same order (the same list when same != 0)
+    const int* bra_seg;          // [nmol + 1] molecule segment starts in bra
+    const int* ket_seg;          // [nmol + 1] molecule segment starts in ket
+    const long long* ket_npcum;  // [nket + 1] cumulative primitive-pair counts along ket
+    const double* bound;         // Schwarz bound per pair id
+    const double* bra_bound;     // bound of bra[i], in bra order
+    const PairHdr* pairs;
+    const int* cart_off;
+    double thr, thr32;
+    int nb, nk, same;
+};
+
+// Per bra of one list (FP64: w = 0, FP32: w = 1).
+struct ListBras {
+    int* j0;          // first ket index in the sorted ket list
+    long long* cnt;   // kets kept
+    long long* np;    // primitive quartets
+    long long* qs;    // first quartet (exclusive scan of cnt)
+    long long* nit;   // J/K work items
+    long long* its;   // first work item (exclusive scan of nit)
+};
+
+// Host launchers (stream-ordered; no synchronisation).
+void list_count(const ListArgs& a, const ListBras& b64, const ListBras& b32, hipStream_t st);
+// Exclusive scan of n values (in may equal out); *total = sum.
+void list_scan(const long long* in, long long* out, int n, long long* total, hipStream_t st);
+void list_items_count(const ListBras& b, int nb, long long run, hipStream_t st);
+void list_fill(const ListArgs& a, const ListBras& b, long long nq, int2* quartets, unsigned* kets, hipStream_t st);
+void list_items_fill(const ListArgs& a, const ListBras& b, long long nitems, long long run, JkItem* items,
+                     hipStream_t st);
+// J'_Q += sum over the bras that reach ket Q of the per-quartet ket J values
+// jkbuf[(tile * ncd + cd) * nq + q] (one thread per ket element).
+void jket_reduce(const ListArgs& a, const ListBras& b, int w, const double* jkbuf, long long nq, int ntiles, int ncd,
+                 int n4, const MolInfo* mols, double* J, hipStream_t st);

@@ -1,0 +1,228 @@
+// lists.hip -- device construction of the screened quartet lists and the ket
+// J reduction (the list layout and its monotonicity are described in lists.h).
+#include <hip/hip_runtime.h>
+
+#include "lists.h"
+
+// First index in [lo, hi) where pred holds, for a predicate that is false
+// then true along the range; hi when it never holds.
+template <typename Pred>
+__device__ __forceinline__ int first_true(int lo, int hi, Pred pred)
+{
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (pred(mid)) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// Largest i in [0, n) with v[i] <= x (v non-decreasing, v[0] <= x).
+__device__ __forceinline__ int last_le(const long long* v, int n, long long x)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = lo + (hi - lo) / 2;
+        if (v[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Per bra: its FP64 ket run [j0, f64) and FP32 run [f64, fa) with
+//   fa  = first ket with !(bound_i bound_j > thr), at least one past the
+//         diagonal when bra and ket lists agree;
+//   f64 = first ket with bound_i bound_j < thr32 (thr32 > 0), clipped to
+//         [j0 + diagonal, fa].
+__global__ void list_count_kernel(const ListArgs a, const ListBras b64, const ListBras b32)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nb) return;
+    const int P = a.bra[i];
+    const PairHdr ph = a.pairs[P];
+    const double qb = a.bound[P];
+    const int e = a.ket_seg[ph.mol + 1];
+    const int j0 = a.same ? i : a.ket_seg[ph.mol];
+    int fa = first_true(j0, e, [&](int j) { return !(qb * a.bound[a.ket[j]] > a.thr); });
+    if (a.same && fa < j0 + 1) fa = j0 + 1;
+    int f64 = a.thr32 > 0.0 ? first_true(j0, e, [&](int j) { return qb * a.bound[a.ket[j]] < a.thr32; }) : e;
+    if (f64 > fa) f64 = fa;
+    if (a.same && f64 < j0 + 1) f64 = j0 + 1;
+    const long long npb = ph.n;
+    b64.j0[i] = j0;
+    b64.cnt[i] = f64 - j0;
+    b64.np[i] = npb * (a.ket_npcum[f64] - a.ket_npcum[j0]);
+    b32.j0[i] = f64;
+    b32.cnt[i] = fa - f64;
+    b32.np[i] = npb * (a.ket_npcum[fa] - a.ket_npcum[f64]);
+}
+
+// Single-workgroup exclusive scan: each thread scans a contiguous chunk,
+// chunk sums are scanned in LDS.
+__global__ void __launch_bounds__(1024) list_scan_kernel(const long long* in, long long* out, int n, long long* total)
+{
+    __shared__ long long part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int lo = min(n, t * per), hi = min(n, lo + per);
+    long long s = 0;
+    for (int i = lo; i < hi; ++i) s += in[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const long long v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long long run = part[t] - s;
+    for (int i = lo; i < hi; ++i) {
+        const long long x = in[i];
+        out[i] = run;
+        run += x;
+    }
+    if (t == 1023) *total = part[1023];
+}
+
+__global__ void list_items_count_kernel(const long long* cnt, long long* nit, int nb, long long run)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb) nit[i] = (cnt[i] + run - 1) / run;
+}
+
+// One thread per quartet: its bra is the last with qs <= q (bras without
+// kets share the next bra's qs and are never last).
+__global__ void list_fill_kernel(const ListArgs a, const ListBras b, long long nq, int2* quartets, unsigned* kets)
+{
+    const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int i = last_le(b.qs, a.nb, q);
+    const int j = b.j0[i] + (int)(q - b.qs[i]);
+    const int P = a.bra[i], Q = a.ket[j];
+    quartets[q] = make_int2(P, Q);
+    const PairHdr kh = a.pairs[Q];
+    kets[q] = (unsigned)a.cart_off[kh.s1] | ((unsigned)a.cart_off[kh.s2] << 15) |
+              ((kh.s1 == kh.s2) ? (1u << 30) : 0u) | ((P == Q) ? (1u << 31) : 0u);
+}
+
+__global__ void list_items_fill_kernel(const ListArgs a, const ListBras b, long long nitems, long long run,
+                                       JkItem* items)
+{
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= nitems) return;
+    const int i = last_le(b.its, a.nb, t);
+    const long long r = t - b.its[i];
+    const long long start = b.qs[i] + r * run;
+    const long long stop = b.qs[i] + b.cnt[i];
+    const PairHdr bh = a.pairs[a.bra[i]];
+    JkItem it;
+    it.start = (int)start;
+    it.end = (int)(start + run < stop ? start + run : stop);
+    it.bra = a.cart_off[bh.s1] | (a.cart_off[bh.s2] << 15) | ((bh.s1 == bh.s2) ? (1 << 30) : 0);
+    it.mol = bh.mol;
+    items[t] = it;
+}
+
+// Ket J reduction in tiles of 64 consecutive kets (one per lane) x 256 bras
+// (blockIdx.y; a quarter per wave) x one ket component (blockIdx.z): for a
+// fixed bra the lanes read neighbouring quartets; the four waves combine in
+// LDS and the tile adds its sums into J' with one atomic per ket element.  The
+// bras reaching ket j are re-derived with the predicates of list_count_kernel,
+// for kets in descending bound order:
+//   FP64: bound_i bound_j > thr and (no split or >= thr32), a prefix of the
+//         molecule's bras (bras below j when the lists agree), plus (j|j);
+//   FP32: thr < bound_i bound_j < thr32, a contiguous range (no diagonal).
+#define JRED_BRAS 256
+__global__ void __launch_bounds__(256) jket_reduce_kernel(const ListArgs a, const ListBras b, int w,
+                                                          const double* jkbuf, long long nq, int ntiles, int ncd,
+                                                          int n4, const MolInfo* mols, double* J)
+{
+    __shared__ double part[3][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + lane;
+    const bool valid = j < a.nk;
+    const int cd = blockIdx.z;
+    const int c0 = blockIdx.y * JRED_BRAS + wv * (JRED_BRAS / 4);   // this wave's bras
+    double sum = 0.0;
+    const int Q = valid ? a.ket[j] : 0;
+    const PairHdr kh = a.pairs[Q];
+    if (valid) {
+        const double qk = a.bound[Q];
+        const int lo = max(a.bra_seg[kh.mol], c0);
+        const int hi = min(a.bra_seg[kh.mol + 1], c0 + JRED_BRAS / 4);
+        const int hnd = a.same ? min(hi, j) : hi;   // non-diagonal bras
+        const bool split = a.thr32 > 0.0;
+        const double* bb = a.bra_bound;
+        // covering bras [r0, r1) of this range (est falls along the bras)
+        int r0 = lo, r1 = lo;
+        if (w == 0) {
+            r1 = first_true(lo, hnd, [&](int i) {
+                const double est = bb[i] * qk;
+                return !(est > a.thr) || (split && est < a.thr32);
+            });
+        } else if (split) {
+            r0 = first_true(lo, hnd, [&](int i) { return bb[i] * qk < a.thr32; });
+            r1 = first_true(r0, hnd, [&](int i) { return !(bb[i] * qk > a.thr); });
+        }
+        auto add = [&](int i) {
+            const long long k = j - b.j0[i];
+            if (k < 0 || k >= b.cnt[i]) return;   // never taken: the predicates match the counts
+            const long long q = b.qs[i] + k;
+            for (int tt = 0; tt < ntiles; ++tt) sum += jkbuf[(size_t)(tt * ncd + cd) * nq + q];
+        };
+#pragma unroll 4
+        for (int i = r0; i < r1; ++i) add(i);
+        if (w == 0 && a.same && j >= lo && j < hi) add(j);
+    }
+    if (wv > 0) part[wv - 1][lane] = sum;
+    __syncthreads();
+    if (wv == 0 && valid) {
+        sum += part[0][lane] + part[1][lane] + part[2][lane];
+        if (sum != 0.0) {
+            const MolInfo mi = mols[kh.mol];
+            const int c = cd / n4, d = cd - (cd / n4) * n4;
+            const int o3 = a.cart_off[kh.s1], o4 = a.cart_off[kh.s2];
+            atomic_addd(&J[mi.base + (size_t)(o3 + c) * mi.n + o4 + d], sum);
+        }
+    }
+}
+
+static inline unsigned blocks_of(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void list_count(const ListArgs& a, const ListBras& b64, const ListBras& b32, hipStream_t st)
+{
+    if (a.nb > 0) hipLaunchKernelGGL(list_count_kernel, dim3(blocks_of(a.nb, 256)), dim3(256), 0, st, a, b64, b32);
+}
+
+void list_scan(const long long* in, long long* out, int n, long long* total, hipStream_t st)
+{
+    hipLaunchKernelGGL(list_scan_kernel, dim3(1), dim3(1024), 0, st, in, out, n, total);
+}
+
+void list_items_count(const ListBras& b, int nb, long long run, hipStream_t st)
+{
+    if (nb > 0)
+        hipLaunchKernelGGL(list_items_count_kernel, dim3(blocks_of(nb, 256)), dim3(256), 0, st, (const long long*)b.cnt,
+                           b.nit, nb, run);
+}
+
+void list_fill(const ListArgs& a, const ListBras& b, long long nq, int2* quartets, unsigned* kets, hipStream_t st)
+{
+    if (nq > 0) hipLaunchKernelGGL(list_fill_kernel, dim3(blocks_of(nq, 256)), dim3(256), 0, st, a, b, nq, quartets, kets);
+}
+
+void list_items_fill(const ListArgs& a, const ListBras& b, long long nitems, long long run, JkItem* items,
+                     hipStream_t st)
+{
+    if (nitems > 0)
+        hipLaunchKernelGGL(list_items_fill_kernel, dim3(blocks_of(nitems, 256)), dim3(256), 0, st, a, b, nitems, run,
+                           items);
+}
+
+void jket_reduce(const ListArgs& a, const ListBras& b, int w, const double* jkbuf, long long nq, int ntiles, int ncd,
+                 int n4, const MolInfo* mols, double* J, hipStream_t st)
+{
+    if (a.nk > 0 && a.nb > 0 && nq > 0)
+        hipLaunchKernelGGL(jket_reduce_kernel, dim3(blocks_of(a.nk, 64), blocks_of(a.nb, JRED_BRAS), ncd), dim3(256), 0,
+                           st, a, b, w, jkbuf, nq, ntiles, ncd, n4, mols, J);
+}

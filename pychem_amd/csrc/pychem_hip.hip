@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/pychem_hip.h"
+#include "lists.h"
 #include "eri_common.h"
 #include "eri_dispatch.h"
 #include "mp2.h"
@@ -165,10 +166,11 @@ struct DevBuf {
 };
 
 struct EriClassStore {
-    std::vector<int2> hq;      // host copy of the quartet list
-    std::vector<int2> hitems;  // J/K work items: [start, end) ranges with one bra pair
-    DevBuf q, vals, items;
+    DevBuf q, vals, items, kets, jkbuf;
+    // per bra of the class's bra list (see lists.h)
+    DevBuf j0, cnt, np, qs, nit, its;
     int nitems = 0;
+    int jk_block = 64;         // J/K threads per block (from the mean item length)
     int jk_run = 0;            // J/K work-item length (0: automatic)
     int jk_grid = 0;           // J/K blocks launched (0: one per item)
     int nq = 0;
@@ -191,6 +193,10 @@ struct pq_basis {
     std::vector<std::vector<int>> class_pairs;   // pair ids per pair class
     std::vector<double> bound;              // per pair Schwarz bound
     int have_bound = 0;
+    // per pair class: pair ids sorted by (molecule, bound desc), molecule
+    // segment starts, cumulative primitive-pair counts (device)
+    DevBuf d_sorted[6], d_seg[6], d_npcum[6], d_sbound[6], d_total;
+    int nsorted[6] = {};
     DevBuf d_pairs, d_prims, d_atoms, d_cart_off, d_bound, d_tmp_list, d_spher, d_mols;
     int any_spher = 0;
     EriClassStore store64[ERI_NCLASS], store32[ERI_NCLASS];
@@ -202,7 +208,22 @@ struct pq_basis {
     hipEvent_t ev_jk[2] = {};
     int ev_jk_used = 0;
     hipGraphExec_t graph_exec = nullptr;
+    // J/K class launches fan out over side streams (default priority)
+    int jk_nstreams = 4;
+    hipStream_t jk_side[4] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[4] = {};
 };
+
+static int ensure_jk_streams(pq_basis* b)
+{
+    if (b->ev_fork) return 0;
+    for (int k = 0; k < 4; ++k) {
+        HIPCHK(hipStreamCreateWithFlags(&b->jk_side[k], hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&b->ev_join[k], hipEventDisableTiming));
+    }
+    HIPCHK(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    return 0;
+}
 
 static int ensure_events(pq_basis* b)
 {
@@ -393,16 +414,31 @@ extern "C" void pq_basis_destroy(pq_basis* b)
     for (int c = 0; c < ERI_NCLASS; ++c) {
         b->store64[c].q.release(); b->store64[c].vals.release(); b->store64[c].items.release();
         b->store32[c].q.release(); b->store32[c].vals.release(); b->store32[c].items.release();
+        for (int w = 0; w < 2; ++w) {
+            EriClassStore& t = w ? b->store32[c] : b->store64[c];
+            t.kets.release(); t.jkbuf.release(); t.j0.release(); t.cnt.release(); t.np.release();
+            t.qs.release(); t.nit.release(); t.its.release();
+        }
+    }
+    for (int X = 0; X < 6; ++X) {
+        b->d_sorted[X].release(); b->d_seg[X].release(); b->d_npcum[X].release(); b->d_sbound[X].release();
     }
     b->d_pairs.release(); b->d_prims.release(); b->d_atoms.release();
     b->d_cart_off.release(); b->d_bound.release(); b->d_tmp_list.release(); b->d_spher.release();
-    b->d_mols.release();
+    b->d_mols.release(); b->d_total.release();
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->ev_jk[0]) {
         for (int i = 0; i < 2 * ERI_NCLASS; ++i)
             for (int k = 0; k < 2; ++k) (void)hipEventDestroy(b->ev_cls[i][k]);
         (void)hipEventDestroy(b->ev_jk[0]);
         (void)hipEventDestroy(b->ev_jk[1]);
+    }
+    if (b->ev_fork) {
+        for (int k = 0; k < 4; ++k) {
+            (void)hipStreamDestroy(b->jk_side[k]);
+            (void)hipEventDestroy(b->ev_join[k]);
+        }
+        (void)hipEventDestroy(b->ev_fork);
     }
     delete b;
 }
@@ -538,82 +574,110 @@ extern "C" int pq_schwarz(pq_basis* b, double* pair_bound_host, void* stream)
     return 0;
 }
 
-// Screened unique quartet lists: for classes X > Y all (bra, ket) with
-// Q_bra*Q_ket > thr; for X == Y the pairs ordered by descending bound and
-// ket rank >= bra rank; diagonal quartets (P|P) always kept (the reference
-// stores them from its bounds pass, Methods/hartree_fock.py:244-273).
-// Quartets with bound >= thr32 (or all if thr32 <= 0) go to the FP64 list.
-static void build_lists(pq_basis* b, double thr, double thr32)
+// Sort every pair class by (molecule, descending bound) on the host (a few
+// thousand pairs) and upload the sorted ids, molecule segments and cumulative
+// primitive-pair counts that the device list builder reads.
+static int upload_sorted_pairs(pq_basis* b, hipStream_t st)
 {
-    // per pair class: pair ids sorted by (molecule, descending bound), plus
-    // the start of every molecule's segment
-    std::vector<std::vector<int>> sorted(6);
-    std::vector<std::vector<int>> seg(6);
     for (int X = 0; X < 6; ++X) {
-        sorted[X] = b->class_pairs[X];
-        std::stable_sort(sorted[X].begin(), sorted[X].end(), [&](int u, int v) {
+        std::vector<int> sorted = b->class_pairs[X];
+        std::stable_sort(sorted.begin(), sorted.end(), [&](int u, int v) {
             if (b->pairs[u].mol != b->pairs[v].mol) return b->pairs[u].mol < b->pairs[v].mol;
             return b->bound[u] > b->bound[v];
         });
-        seg[X].assign(b->nmol + 1, 0);
-        for (int id : sorted[X]) seg[X][b->pairs[id].mol + 1]++;
-        for (int m = 0; m < b->nmol; ++m) seg[X][m + 1] += seg[X][m];
+        std::vector<int> seg(b->nmol + 1, 0);
+        for (int id : sorted) seg[b->pairs[id].mol + 1]++;
+        for (int m = 0; m < b->nmol; ++m) seg[m + 1] += seg[m];
+        std::vector<long long> npcum(sorted.size() + 1, 0);
+        for (size_t k = 0; k < sorted.size(); ++k) npcum[k + 1] = npcum[k] + b->pairs[sorted[k]].n;
+        std::vector<double> sb(std::max<size_t>(1, sorted.size()), 0.0);
+        for (size_t k = 0; k < sorted.size(); ++k) sb[k] = b->bound[sorted[k]];
+        const int n = (int)sorted.size();
+        b->nsorted[X] = n;
+        if (b->d_sorted[X].ensure(sizeof(int) * std::max(1, n)) || b->d_seg[X].ensure(sizeof(int) * seg.size()) ||
+            b->d_npcum[X].ensure(sizeof(long long) * npcum.size()) || b->d_sbound[X].ensure(sizeof(double) * sb.size()))
+            return -1;
+        HIPCHK(hipMemcpyAsync(b->d_sbound[X].p, sb.data(), sizeof(double) * sb.size(), hipMemcpyHostToDevice, st));
+        if (n) HIPCHK(hipMemcpyAsync(b->d_sorted[X].p, sorted.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(b->d_seg[X].p, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(b->d_npcum[X].p, npcum.data(), sizeof(long long) * npcum.size(), hipMemcpyHostToDevice,
+                              st));
+        // the host vectors go out of scope: wait for the copies
+        HIPCHK(hipStreamSynchronize(st));
     }
-    for (int c = 0; c < ERI_NCLASS; ++c) {
-        b->store64[c].hq.clear(); b->store32[c].hq.clear();
-        b->store64[c].nprimq = b->store32[c].nprimq = 0;
-    }
-    for (int c = 0; c < ERI_NCLASS; ++c) {
-        const int X = eri_class_table[c][5], Y = eri_class_table[c][6];
-        const std::vector<int>& bl = sorted[X];
-        const std::vector<int>& kl = sorted[Y];
-        EriClassStore& s64 = b->store64[c];
-        EriClassStore& s32 = b->store32[c];
-        for (size_t i = 0; i < bl.size(); ++i) {
-            const int P = bl[i];
-            const int m = b->pairs[P].mol;
-            const double qb = b->bound[P];
-            const size_t j0 = (X == Y) ? i : (size_t)seg[Y][m];
-            const size_t j1 = (size_t)seg[Y][m + 1];
-            for (size_t j = j0; j < j1; ++j) {
-                const int Q = kl[j];
-                const double est = qb * b->bound[Q];
-                const bool diag = (X == Y && j == i);
-                if (!(est > thr) && !diag) break;   // kets sorted by descending bound
-                const long long np = (long long)b->pairs[P].n * b->pairs[Q].n;
-                if (thr32 > 0.0 && est < thr32 && !diag) {
-                    s32.hq.push_back(make_int2(P, Q));
-                    s32.nprimq += np;
-                } else {
-                    s64.hq.push_back(make_int2(P, Q));
-                    s64.nprimq += np;
-                }
-            }
-        }
-    }
+    return 0;
 }
 
-// J/K work items: runs of equal bra pair, cut into pieces of at most jk_run
-// quartets so that long runs spread over several waves.  Automatic length:
-// shrink for classes with few bra pairs so that every class still launches
-// ~2048 waves (8 per CU).
-static int make_items(EriClassStore& s)
+static ListArgs list_args(const pq_basis* b, int c)
 {
-    int run = s.jk_run;
+    const int X = eri_class_table[c][5], Y = eri_class_table[c][6];
+    ListArgs a;
+    a.bra = (const int*)b->d_sorted[X].p;
+    a.ket = (const int*)b->d_sorted[Y].p;
+    a.bra_seg = (const int*)b->d_seg[X].p;
+    a.ket_seg = (const int*)b->d_seg[Y].p;
+    a.ket_npcum = (const long long*)b->d_npcum[Y].p;
+    a.bound = (const double*)b->d_bound.p;
+    a.bra_bound = (const double*)b->d_sbound[X].p;
+    a.pairs = (const PairHdr*)b->d_pairs.p;
+    a.cart_off = (const int*)b->d_cart_off.p;
+    a.thr = b->threshold;
+    a.thr32 = b->threshold32;
+    a.nb = b->nsorted[X];
+    a.nk = b->nsorted[Y];
+    a.same = X == Y;
+    return a;
+}
+
+static ListBras list_bras(EriClassStore& s)
+{
+    ListBras r;
+    r.j0 = (int*)s.j0.p;
+    r.cnt = (long long*)s.cnt.p;
+    r.np = (long long*)s.np.p;
+    r.qs = (long long*)s.qs.p;
+    r.nit = (long long*)s.nit.p;
+    r.its = (long long*)s.its.p;
+    return r;
+}
+
+// J/K work items of one list on the device: runs of equal bra pair cut into
+// pieces of at most jk_run quartets.  Automatic length: shrink for classes
+// with few bra pairs so that every class still launches ~2048 blocks.  The
+// block size follows the mean item length (64..256).  Synchronises `st` to
+// learn the item count.
+static int build_items(pq_basis* b, int c, int w, hipStream_t st)
+{
+    EriClassStore& s = w ? b->store32[c] : b->store64[c];
+    s.nitems = 0;
+    if (!s.nq) return 0;
+    long long run = s.jk_run;
     if (run <= 0) {
         run = s.nq / 2048;
         run = run < 64 ? 64 : (run > 1024 ? 1024 : run);
     }
-    s.hitems.clear();
-    int start = 0;
-    for (int i = 1; i <= s.nq; ++i) {
-        if (i == s.nq || s.hq[i].x != s.hq[start].x || i - start == run) {
-            s.hitems.push_back(make_int2(start, i));
-            start = i;
-        }
-    }
-    s.nitems = (int)s.hitems.size();
-    return s.items.ensure(sizeof(int2) * std::max(1, s.nitems));
+    const ListArgs a = list_args(b, c);
+    const ListBras lb = list_bras(s);
+    list_items_count(lb, a.nb, run, st);
+    list_scan(lb.nit, lb.its, a.nb, (long long*)b->d_total.p, st);
+    long long nitems = 0;
+    HIPCHK(hipMemcpyAsync(&nitems, b->d_total.p, sizeof(long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (nitems > 0x7fffffffLL) return fail("J/K work items exceed 2^31");
+    s.nitems = (int)nitems;
+    if (s.items.ensure(sizeof(JkItem) * std::max(1, s.nitems))) return -1;
+    list_items_fill(a, lb, nitems, run, (JkItem*)s.items.p, st);
+    const double mean = s.nitems ? (double)s.nq / s.nitems : 0.0;
+    s.jk_block = mean > 160.0 ? 256 : (mean > 80.0 ? 128 : 64);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// The tiled kernel packs offsets into 15 bits and keeps 2 (TA + TB) rows of
+// the largest molecule in LDS (at most 64 KiB).
+static inline bool jk_tile_fits(const pq_basis* b, int cls)
+{
+    return b->nmax < 32768 && sizeof(double) * (size_t)jk_tile_table[cls][3] * b->nmax <= 65536;
 }
 
 static inline int jk_grid_of(const EriClassStore& s)
@@ -628,19 +692,61 @@ extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold,
     if (!b->have_bound && pq_schwarz(b, nullptr, stream)) return -1;
     b->threshold = threshold;
     b->threshold32 = fp32_threshold;
-    build_lists(b, threshold, fp32_threshold);
+    // Screened unique quartet lists, built on the device (lists.h): for
+    // classes X != Y every (bra, ket) of a molecule with Q_bra Q_ket > thr;
+    // for X == Y ket rank >= bra rank; diagonal quartets (P|P) always kept
+    // (the reference stores them from its bounds pass,
+    // Methods/hartree_fock.py:244-273).  Quartets with bound product below
+    // thr32 (when > 0) go to the FP32 list.
+    if (upload_sorted_pairs(b, st)) return -1;
+    if (b->d_total.ensure(sizeof(long long) * 4 * ERI_NCLASS)) return -1;
+    long long* dtot = (long long*)b->d_total.p;
     for (int c = 0; c < ERI_NCLASS; ++c) {
+        const ListArgs a = list_args(b, c);
+        const size_t nb = (size_t)std::max(1, a.nb);
         for (int w = 0; w < 2; ++w) {
             EriClassStore& s = w ? b->store32[c] : b->store64[c];
-            s.nq = (int)s.hq.size();
-            if (!s.nq) continue;
+            if (s.j0.ensure(sizeof(int) * nb) || s.cnt.ensure(8 * nb) || s.np.ensure(8 * nb) || s.qs.ensure(8 * nb) ||
+                s.nit.ensure(8 * nb) || s.its.ensure(8 * nb))
+                return -1;
+        }
+        if (a.nb == 0 || a.nk == 0) {
+            hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(256), 0, st, (double*)(dtot + 4 * c), 4LL,
+                               (double*)nullptr);
+            continue;
+        }
+        const ListBras l64 = list_bras(b->store64[c]), l32 = list_bras(b->store32[c]);
+        list_count(a, l64, l32, st);
+        list_scan(l64.cnt, l64.qs, a.nb, dtot + 4 * c + 0, st);
+        list_scan(l32.cnt, l32.qs, a.nb, dtot + 4 * c + 1, st);
+        list_scan(l64.np, l64.nit, a.nb, dtot + 4 * c + 2, st);   // nit as scratch: only the totals are kept
+        list_scan(l32.np, l32.nit, a.nb, dtot + 4 * c + 3, st);
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<long long> tot(4 * ERI_NCLASS);
+    HIPCHK(hipMemcpyAsync(tot.data(), dtot, sizeof(long long) * tot.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        const ListArgs a = list_args(b, c);
+        const int ncd = ncart_l(eri_class_table[c][2]) * ncart_l(eri_class_table[c][3]);
+        for (int w = 0; w < 2; ++w) {
+            EriClassStore& s = w ? b->store32[c] : b->store64[c];
+            const long long nq = tot[4 * c + w];
+            if (nq > 0x7fffffffLL) return fail("pq_eri_plan: more than 2^31 quartets in one class");
+            s.nq = (int)nq;
+            s.nprimq = tot[4 * c + 2 + w];
+            if (!s.nq) { s.nitems = 0; continue; }
             const size_t vb = (size_t)s.nq * s.ncomp * (w ? sizeof(float) : sizeof(double));
-            if (s.q.ensure(sizeof(int2) * s.nq) || s.vals.ensure(vb)) return -1;
-            HIPCHK(hipMemcpyAsync(s.q.p, s.hq.data(), sizeof(int2) * s.nq, hipMemcpyHostToDevice, st));
-            if (make_items(s)) return -1;
-            HIPCHK(hipMemcpyAsync(s.items.p, s.hitems.data(), sizeof(int2) * s.nitems, hipMemcpyHostToDevice, st));
+            const size_t jb = jk_tile_fits(b, c) ? sizeof(double) * (size_t)s.nq * jk_tile_table[c][2] * ncd : 0;
+            if (s.q.ensure(sizeof(int2) * s.nq) || s.vals.ensure(vb) || s.kets.ensure(sizeof(unsigned) * s.nq) ||
+                s.jkbuf.ensure(jb))
+                return -1;
+            const ListBras lb = list_bras(s);
+            list_fill(a, lb, s.nq, (int2*)s.q.p, (unsigned*)s.kets.p, st);
+            if (build_items(b, c, w, st)) return -1;
         }
     }
+    HIPCHK(hipGetLastError());
     return 0;
 }
 
@@ -746,6 +852,43 @@ extern "C" int pq_eri_stats(const pq_basis* b, long long* stats, int nstats)
 
 extern "C" int pq_eri_nclass(void) { return ERI_NCLASS; }
 
+// One stored class (w = 0: FP64 list, 1: FP32 list) into J', K'.
+static int jk_launch_class(pq_basis* b, int c, int w, const double* Dj, const double* Dk, double* J, double* K,
+                           hipStream_t st)
+{
+    EriClassStore& s = w ? b->store32[c] : b->store64[c];
+    if (!s.nq) return 0;
+    const MolInfo* dm = (const MolInfo*)b->d_mols.p;
+    if (jk_tile_fits(b, c)) {
+        double* jb = (double*)s.jkbuf.p;
+        if (w) {
+            JkTileArgs<float> a{(const JkItem*)s.items.p, (const unsigned*)s.kets.p, dm, (const float*)s.vals.p,
+                                Dj, Dk, J, K, jb, s.nq, b->nmax, s.nitems, jk_grid_of(s)};
+            jktile_launch<float>(c, a, s.jk_block, st);
+        } else {
+            JkTileArgs<double> a{(const JkItem*)s.items.p, (const unsigned*)s.kets.p, dm, (const double*)s.vals.p,
+                                 Dj, Dk, J, K, jb, s.nq, b->nmax, s.nitems, jk_grid_of(s)};
+            jktile_launch<double>(c, a, s.jk_block, st);
+        }
+        const int n3 = ncart_l(eri_class_table[c][2]), n4 = ncart_l(eri_class_table[c][3]);
+        jket_reduce(list_args(b, c), list_bras(s), w, jb, s.nq, jk_tile_table[c][2], n3 * n4, n4, dm, J, st);
+    } else if (w) {
+        JkArgs<float> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p, dm,
+                        (const float*)s.vals.p, Dj, Dk, J, K, s.nq, b->nmax};
+        jk_launch<float>(c, a, st);
+    } else {
+        JkArgs<double> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p, dm,
+                         (const double*)s.vals.p, Dj, Dk, J, K, s.nq, b->nmax};
+        jk_launch<double>(c, a, st);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// Zero J', K'; digest every stored class; symmetrise.  With jk_nstreams > 1
+// the class launches fork from `st` onto side streams (largest stored bytes
+// first, each to the least-loaded stream) and join back before the final
+// kernel, so that small high-l classes overlap the large ones.
 static int jk_launch_all(pq_basis* b, const double* Dj, const double* Dk, double* J, double* K, hipStream_t st)
 {
     const long long tot = b->mat_total;
@@ -753,41 +896,46 @@ static int jk_launch_all(pq_basis* b, const double* Dj, const double* Dk, double
     blocks = blocks > 4096 ? 4096 : (blocks < 1 ? 1 : blocks);
     hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, st, J, tot, K);
     const MolInfo* dm = (const MolInfo*)b->d_mols.p;
-    for (int c = 0; c < ERI_NCLASS; ++c) {
-        const int nbra = ncart_l(eri_class_table[c][0]) + ncart_l(eri_class_table[c][1]);
-        const bool use_runs = sizeof(double) * (size_t)nbra * b->nmax <= 65536;
+    std::vector<std::pair<double, int>> order;
+    for (int c = 0; c < ERI_NCLASS; ++c)
         for (int w = 0; w < 2; ++w) {
-            EriClassStore& s = w ? b->store32[c] : b->store64[c];
-            if (!s.nq) continue;
-            if (use_runs) {
-                if (w) {
-                    JkRunArgs<float> a{(const int2*)s.q.p, (const int2*)s.items.p, (const PairHdr*)b->d_pairs.p,
-                                       (const int*)b->d_cart_off.p, dm, (const float*)s.vals.p, Dj, Dk, J, K, s.nq,
-                                       b->nmax, s.nitems, jk_grid_of(s)};
-                    jkrun_launch<float>(c, a, st);
-                } else {
-                    JkRunArgs<double> a{(const int2*)s.q.p, (const int2*)s.items.p, (const PairHdr*)b->d_pairs.p,
-                                        (const int*)b->d_cart_off.p, dm, (const double*)s.vals.p, Dj, Dk, J, K, s.nq,
-                                        b->nmax, s.nitems, jk_grid_of(s)};
-                    jkrun_launch<double>(c, a, st);
-                }
-                continue;
-            }
-            if (w) {
-                JkArgs<float> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p, dm,
-                                (const float*)s.vals.p, Dj, Dk, J, K, s.nq, b->nmax};
-                jk_launch<float>(c, a, st);
-            } else {
-                JkArgs<double> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p, dm,
-                                 (const double*)s.vals.p, Dj, Dk, J, K, s.nq, b->nmax};
-                jk_launch<double>(c, a, st);
-            }
+            const EriClassStore& s = w ? b->store32[c] : b->store64[c];
+            if (s.nq) order.push_back({(double)s.nq * s.ncomp * (w ? 4.0 : 8.0) + 16.0 * s.nq, 2 * c + w});
         }
+    std::sort(order.begin(), order.end(), [](const std::pair<double, int>& u, const std::pair<double, int>& v) {
+        return u.first > v.first;
+    });
+    const int ns = std::min(4, std::max(1, b->jk_nstreams));
+    if (ns > 1) {
+        if (ensure_jk_streams(b)) return -1;
+        HIPCHK(hipEventRecord(b->ev_fork, st));
+        for (int k = 0; k < ns; ++k) HIPCHK(hipStreamWaitEvent(b->jk_side[k], b->ev_fork, 0));
     }
+    double load[4] = {0.0, 0.0, 0.0, 0.0};
+    for (const auto& o : order) {
+        int k = 0;
+        for (int t = 1; t < ns; ++t)
+            if (load[t] < load[k]) k = t;
+        load[k] += o.first;
+        if (jk_launch_class(b, o.second / 2, o.second % 2, Dj, Dk, J, K, ns > 1 ? b->jk_side[k] : st)) return -1;
+    }
+    if (ns > 1)
+        for (int k = 0; k < ns; ++k) {
+            HIPCHK(hipEventRecord(b->ev_join[k], b->jk_side[k]));
+            HIPCHK(hipStreamWaitEvent(st, b->ev_join[k], 0));
+        }
     int gx = (int)((b->nmax * (long long)b->nmax + 255) / 256);
     gx = gx < 1 ? 1 : (gx > 64 ? 64 : gx);
     hipLaunchKernelGGL(jk_finish_kernel, dim3(gx, b->nmol), dim3(256), 0, st, J, K, dm);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// Number of streams the J/K class launches spread over (1..4).
+extern "C" int pq_jk_set_streams(pq_basis* b, int nstreams)
+{
+    if (!b || nstreams < 1 || nstreams > 4) return fail("pq_jk_set_streams: need a basis and 1..4 streams");
+    b->jk_nstreams = nstreams;
     return 0;
 }
 
@@ -802,8 +950,7 @@ extern "C" int pq_jk_set_params(pq_basis* b, int cls, int run_len, int grid, voi
         s.jk_run = run_len;
         s.jk_grid = grid;
         if (!s.nq) continue;
-        if (make_items(s)) return -1;
-        HIPCHK(hipMemcpyAsync(s.items.p, s.hitems.data(), sizeof(int2) * s.nitems, hipMemcpyHostToDevice, st));
+        if (build_items(b, cls, w, st)) return -1;
     }
     HIPCHK(hipStreamSynchronize(st));
     return 0;
@@ -816,18 +963,10 @@ extern "C" int pq_jk_class(pq_basis* b, int cls, const double* Dj, const double*
 {
     if (!b || cls < 0 || cls >= ERI_NCLASS || !Dj || !Dk || !J || !K) return fail("pq_jk_class: bad argument");
     hipStream_t st = (hipStream_t)stream;
-    const MolInfo* dm = (const MolInfo*)b->d_mols.p;
-    const int nbra = ncart_l(eri_class_table[cls][0]) + ncart_l(eri_class_table[cls][1]);
-    if (sizeof(double) * (size_t)nbra * b->nmax > 65536) return fail("pq_jk_class: LDS rows too large for this basis");
+    if (!jk_tile_fits(b, cls)) return fail("pq_jk_class: LDS rows too large for this basis");
     EriClassStore& s = b->store64[cls];
     if (nitems) *nitems = s.nitems;
-    if (!s.nq) return 0;
-    JkRunArgs<double> a{(const int2*)s.q.p, (const int2*)s.items.p, (const PairHdr*)b->d_pairs.p,
-                        (const int*)b->d_cart_off.p, dm, (const double*)s.vals.p, Dj, Dk, J, K, s.nq, b->nmax,
-                        s.nitems, jk_grid_of(s)};
-    jkrun_launch<double>(cls, a, st);
-    HIPCHK(hipGetLastError());
-    return 0;
+    return jk_launch_class(b, cls, 0, Dj, Dk, J, K, st);
 }
 
 extern "C" int pq_jk(pq_basis* b, const double* Dj, const double* Dk, double* J, double* K, void* stream)
