@@ -9,11 +9,17 @@ guess (atomic SCFs included), SCF with Pulay DIIS to |dE| < 1e-10.
 
 Prints ONE JSON line.  ``value`` = screened unique ERI shell quartets per
 second of whole-solve wall time (quartets computed per solve / seconds per
-solve).  ``roofline`` describes the dominant ERI class kernel, timed live with
-HIP events on the stream it runs on; ``cpu_baseline`` is the CPU oracle
-(oracle/oracle.c, OpenMP) running the same RHF on the host cores.
+solve).  ``roofline`` describes the dominant kernel of the solve: the J/K
+digestion of the costliest ERI class (one launch per SCF iteration), HBM
+bound, timed live with HIP events on the stream it runs on; its ``traffic``
+comes from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
+profiles/pmc_traffic.json when they were taken for that kernel.
+``eri_roofline`` gives the ERI generation pass against the FP64 vector peak.
+``cpu_baseline`` is the CPU oracle (oracle/oracle.c, OpenMP) running the same
+RHF on the host cores.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--workload c10h22|c4h10|water]
+    python bench.py --copy-calibration MIB   # plain device copy, for PMC byte-counter calibration
 """
 import argparse
 import json
@@ -57,6 +63,77 @@ def gpu_solve(mol_factory, timing=True, guess_kind="SAD", fp32_threshold=0.0, ke
         return r, stats, mol
     mol.Engine.close()
     return r, stats
+
+
+def jk_roofline(mol, reps=10):
+    """Time the tiled J/K digestion kernel of every FP64 class alone
+    (pq_jk_class part 1) with HIP events on the current stream and price its
+    algorithmic HBM bytes per launch: stored ERI values nq*ncomp*8, packed ket
+    records nq*4, work items nitems*16, ket J values written nq*ntiles*N3*N4*8.
+    Returns the per-class table and the dominant (slowest) class."""
+    import ctypes
+    import torch
+    from pychem_amd import _lib
+    from pychem_amd.engine import _ptr, _stream
+    L = _lib.lib()
+    eng = mol.Engine
+    fm = flop_model()
+    n = eng.ncart
+    D = eng.to_cart(mol.State.Alpha.Density.to("cuda")).contiguous()
+    J = torch.zeros(n, n, dtype=torch.float64, device="cuda")
+    K = torch.zeros_like(J)
+    out = []
+    for s in eng.stats():
+        if not s["nq64"]:
+            continue
+        cls = s["index"]
+        nit = ctypes.c_int(0)
+        _lib.check(L.pq_jk_class(eng.h, cls, 1, _ptr(D), _ptr(D), _ptr(J), _ptr(K), ctypes.byref(nit), _stream()),
+                   "pq_jk_class")
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            _lib.check(L.pq_jk_class(eng.h, cls, 1, _ptr(D), _ptr(D), _ptr(J), _ptr(K), None, _stream()), "pq_jk_class")
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        la, lb, lc, ld = (int(x) for x in s["cls"])
+        ta, tb, nt = fm[s["cls"]]["jk_tile"]
+        ncd = (lc + 1) * (lc + 2) // 2 * ((ld + 1) * (ld + 2) // 2)
+        nq = s["nq64"]
+        nbytes = nq * s["ncomp"] * 8 + nq * 4 + nit.value * 16 + nq * nt * ncd * 8
+        out.append(dict(cls=s["cls"], kernel="jk_tile_kernel<%d, %d, %d, %d, %d, %d, double>" % (la, lb, lc, ld, ta, tb),
+                        nq=nq, items=nit.value, bytes=nbytes, ms=ms, gbs=nbytes / (ms * 1e-3) / 1e9))
+    dom = max(out, key=lambda c: c["ms"])
+    return out, dom
+
+
+def pmc_traffic(kernel, workload):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy-kernel
+    calibration), or None when it was taken for another kernel or workload."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    if d.get("kernel") != kernel or d.get("workload") != workload:
+        return None
+    return d["fetch_bytes"] + d["write_bytes"]
+
+
+def copy_calibration(mib):
+    """One plain device-to-device copy of ``mib`` MiB, repeated 5 times: the
+    known-size reference for the PMC byte counters."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}))
 
 
 def mp2_measure(mol, r):
@@ -131,6 +208,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-size", type=int, default=512)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -141,6 +219,9 @@ def main():
     from pychem_amd import _lib
     if not torch.cuda.is_available():
         raise _lib.HipError("no GPU visible")
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     _lib.lib()
 
     def factory():
@@ -161,6 +242,7 @@ def main():
     r0, stats = results[-1]
     # MP2 on the same molecule (outside the timed RHF steps)
     rk, _, molk = gpu_solve(factory, keep=True)
+    jk_table, jk_dom = jk_roofline(molk)
     mp2_info = mp2_measure(molk, rk)
     molk.Engine.close()
     # mixed precision: Schwarz-split FP32 ERIs below a bound of 1e-6
@@ -189,12 +271,20 @@ def main():
         per_class.append(dict(cls=s["cls"], nq=s["nq64"], nprim=s["nprim64"], ms=round(float(ms), 4),
                               alg_gflop=alg / 1e9, tflops=(alg / (ms * 1e-3) / 1e12) if ms > 0 else 0.0))
     dom = max(per_class, key=lambda c: c["ms"])
-    roofline = {"bound": "fp64-valu", "kernel": "eri_k%s<double>" % dom["cls"],
-                "achieved": round(dom["tflops"], 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(dom["tflops"] / FP64_PEAK_TFLOPS, 5), "traffic": None,
-                "alg_flop_per_launch": dom["alg_gflop"] * 1e9, "launch_ms": dom["ms"],
-                "eri_pass": {"ms": round(eri_pass_ms, 4), "alg_tflops": round(tot_alg / (eri_pass_ms * 1e-3) / 1e12, 4),
-                             "frac": round(tot_alg / (eri_pass_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5)}}
+    eri_roofline = {"bound": "fp64-valu", "kernel": "eri_k%s<double>" % dom["cls"],
+                    "achieved": round(dom["tflops"], 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(dom["tflops"] / FP64_PEAK_TFLOPS, 5),
+                    "alg_flop_per_launch": dom["alg_gflop"] * 1e9, "launch_ms": dom["ms"],
+                    "eri_pass": {"ms": round(eri_pass_ms, 4),
+                                 "alg_tflops": round(tot_alg / (eri_pass_ms * 1e-3) / 1e12, 4),
+                                 "frac": round(tot_alg / (eri_pass_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5)}}
+    jk_gbs = jk_dom["gbs"]
+    roofline = {"bound": "hbm", "kernel": jk_dom["kernel"], "achieved": round(jk_gbs, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(jk_gbs / HBM_PEAK_GBS, 5),
+                "traffic": pmc_traffic(jk_dom["kernel"], cfg["workload"]),
+                "alg_bytes_per_launch": jk_dom["bytes"], "launch_ms": round(jk_dom["ms"], 5),
+                "jk_pass_classes": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in c.items()
+                                     if k != "kernel"} for c in jk_table]}
     out = {
         "metric": "RHF time-to-solution & ERI quartets/s: plain FP64 vs mixed+learned, 1 GPU",
         "value": value, "unit": "ERI shell quartets/s (whole RHF solve)",
@@ -209,6 +299,7 @@ def main():
         "fock_builds_per_s": 1e3 / jk_avg if jk_avg > 0 else None, "jk_ms": jk_avg,
         "stage_s": {k: v for k, v in r0["timings"].items() if isinstance(v, float)},
         "roofline": roofline,
+        "eri_roofline": eri_roofline,
         "per_class": per_class,
         "mp2": mp2_info,
         "mixed": {"time_to_solution_s": mixed_s, "speedup_vs_fp64": (ms_per_step / 1e3) / mixed_s,

@@ -114,10 +114,12 @@ int pq_jk(pq_basis* b, const double* Dj_dev, const double* Dk_dev, double* J_dev
 /* Autotuning hooks of the J/K digestion: per class, work-item length
  * (run_len, 0 = automatic) and number of blocks (grid, 0 = one per item,
  * smaller = persistent blocks looping over items); and a single-class digest
- * into unsymmetrised J', K' (nitems receives the work-item count). */
+ * of the FP64 list into unsymmetrised J', K' (nitems receives the work-item
+ * count).  part: 1 = tiled digestion kernel only, 2 = ket-side J reduction
+ * only (reads what part 1 stored), 3 = both. */
 int pq_jk_set_params(pq_basis* b, int cls, int run_len, int grid, void* stream);
-int pq_jk_class(pq_basis* b, int cls, const double* Dj_dev, const double* Dk_dev, double* J_dev, double* K_dev,
-                int* nitems, void* stream);
+int pq_jk_class(pq_basis* b, int cls, int part, const double* Dj_dev, const double* Dk_dev, double* J_dev,
+                double* K_dev, int* nitems, void* stream);
 /* Number of streams (1..4, default 4) that pq_jk spreads its class launches
  * over, forked from and joined back to the caller's stream (no reference
  * counterpart: the reference digests one quartet at a time on the host). */

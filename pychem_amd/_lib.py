@@ -28,7 +28,7 @@ SIGNATURES = {
                                       c_int, P_double, P_double, P_int, ctypes.POINTER(c_void_p)]),
     "pq_basis_molecules": (c_int, [c_void_p, P_ll, P_int, P_int, P_int]),
     "pq_jk_set_params": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
-    "pq_jk_class": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, P_int, c_void_p]),
+    "pq_jk_class": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, P_int, c_void_p]),
     "pq_jk_graph_create": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pq_jk_graph_launch": (c_int, [c_void_p, c_void_p]),
     "pq_jk_set_streams": (c_int, [c_void_p, c_int]),

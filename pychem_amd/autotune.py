@@ -23,12 +23,12 @@ GRIDS = (0, 256, 512, 1024, 2048, 4096)         # 0 = one block per work item
 def _time_class(h, cls, Dj, Dk, J, K, reps=3):
     L = _lib.lib()
     nitems = ctypes.c_int(0)
-    check(L.pq_jk_class(h, cls, _ptr(Dj), _ptr(Dk), _ptr(J), _ptr(K), ctypes.byref(nitems), _stream()), "pq_jk_class")
+    check(L.pq_jk_class(h, cls, 3, _ptr(Dj), _ptr(Dk), _ptr(J), _ptr(K), ctypes.byref(nitems), _stream()), "pq_jk_class")
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     t0.record()
     for _ in range(reps):
-        check(L.pq_jk_class(h, cls, _ptr(Dj), _ptr(Dk), _ptr(J), _ptr(K), None, _stream()), "pq_jk_class")
+        check(L.pq_jk_class(h, cls, 3, _ptr(Dj), _ptr(Dk), _ptr(J), _ptr(K), None, _stream()), "pq_jk_class")
     t1.record()
     torch.cuda.synchronize()
     return t0.elapsed_time(t1) / reps, nitems.value

@@ -377,7 +377,7 @@ def main():
         tag = "%d%d%d%d" % (la, lb, lc, ld)
         code, ncomp, nbody = gen_eri_class(la, lb, lc, ld)
         table.append((la, lb, lc, ld, ncomp, X, Y, nbody))
-        flop_table[tag] = gen_eri_class.flops
+        flop_table[tag] = dict(gen_eri_class.flops, jk_tile=list(jk_tile(la, lb, lc, ld)))
         src = ["// generated by gen_eri.py -- do not edit", '#include "../eri_common.h"', code]
         jsrc = ["// generated by gen_eri.py -- do not edit", '#include "../jk_common.h"']
         for rn, rt in (("d", "double"), ("f", "float")):

@@ -46,6 +46,20 @@ void list_fill(const ListArgs& a, const ListBras& b, long long nq, int2* quartet
 void list_items_fill(const ListArgs& a, const ListBras& b, long long nitems, long long run, JkItem* items,
                      hipStream_t st);
 // J'_Q += sum over the bras that reach ket Q of the per-quartet ket J values
-// jkbuf[(tile * ncd + cd) * nq + q] (one thread per ket element).
+// jkbuf[(tile * ncd + cd) * nq + q]; one list per launch, or every list of a
+// table in one launch.
+struct JredList {
+    ListArgs a;
+    ListBras b;
+    const double* jkbuf;
+    long long nq;
+    int w, ntiles, ncd, n4;
+    int bx, by;          // blocks over kets (64 each) and bras (256 each); ncd planes
+    long long block0;    // first block of this list in the fused launch
+};
+JredList jred_list(const ListArgs& a, const ListBras& b, int w, const double* jkbuf, long long nq, int ntiles, int ncd,
+                   int n4);
+void jket_reduce_all(const JredList* lists_dev, int nlist, long long nblocks, const MolInfo* mols, double* J,
+                     hipStream_t st);
 void jket_reduce(const ListArgs& a, const ListBras& b, int w, const double* jkbuf, long long nq, int ntiles, int ncd,
                  int n4, const MolInfo* mols, double* J, hipStream_t st);

@@ -124,55 +124,69 @@ __global__ void list_items_fill_kernel(const ListArgs a, const ListBras b, long 
 }
 
 // Ket J reduction in tiles of 64 consecutive kets (one per lane) x 256 bras
-// (blockIdx.y; a quarter per wave) x one ket component (blockIdx.z): for a
-// fixed bra the lanes read neighbouring quartets; the four waves combine in
-// LDS and the tile adds its sums into J' with one atomic per ket element.  The
-// bras reaching ket j are re-derived with the predicates of list_count_kernel,
-// for kets in descending bound order:
+// (blockIdx.y; a quarter per wave) x one ket component (blockIdx.z).  The
+// tile's bra data (first quartet minus first ket, ket range, bound) is staged
+// in LDS, so that each bra costs one global load per lane: the value of the
+// lane's ket under that bra, and neighbouring lanes read neighbouring values.
+// The four waves combine in LDS and the tile adds its sums into J' with one
+// atomic per ket element.  The bras reaching ket j are re-derived with the
+// predicates of list_count_kernel, for kets in descending bound order:
 //   FP64: bound_i bound_j > thr and (no split or >= thr32), a prefix of the
 //         molecule's bras (bras below j when the lists agree), plus (j|j);
 //   FP32: thr < bound_i bound_j < thr32, a contiguous range (no diagonal).
 #define JRED_BRAS 256
-__global__ void __launch_bounds__(256) jket_reduce_kernel(const ListArgs a, const ListBras b, int w,
-                                                          const double* jkbuf, long long nq, int ntiles, int ncd,
-                                                          int n4, const MolInfo* mols, double* J)
+__device__ __forceinline__ void jket_reduce_tile(const ListArgs& a, const ListBras& b, int w, const double* jkbuf,
+                                                 long long nq, int ntiles, int ncd, int n4, const MolInfo* mols,
+                                                 double* J, int bxk, int byb, int cd)
 {
+    __shared__ long long s_base[JRED_BRAS];
+    __shared__ double s_bound[JRED_BRAS];
+    __shared__ int s_j0[JRED_BRAS], s_cnt[JRED_BRAS];
     __shared__ double part[3][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int j = blockIdx.x * 64 + lane;
+    const int cb = byb * JRED_BRAS;   // first bra of the tile
+    const int nbt = min(JRED_BRAS, a.nb - cb);
+    for (int t = threadIdx.x; t < nbt; t += blockDim.x) {
+        const int i = cb + t;
+        s_j0[t] = b.j0[i];
+        s_cnt[t] = (int)b.cnt[i];
+        s_base[t] = b.qs[i] - b.j0[i];
+        s_bound[t] = a.bra_bound[i];
+    }
+    __syncthreads();
+    const int j = bxk * 64 + lane;
     const bool valid = j < a.nk;
-    const int cd = blockIdx.z;
-    const int c0 = blockIdx.y * JRED_BRAS + wv * (JRED_BRAS / 4);   // this wave's bras
     double sum = 0.0;
     const int Q = valid ? a.ket[j] : 0;
     const PairHdr kh = a.pairs[Q];
     if (valid) {
         const double qk = a.bound[Q];
-        const int lo = max(a.bra_seg[kh.mol], c0);
-        const int hi = min(a.bra_seg[kh.mol + 1], c0 + JRED_BRAS / 4);
-        const int hnd = a.same ? min(hi, j) : hi;   // non-diagonal bras
+        // this wave's bras, as tile-local indices
+        const int w0 = wv * (JRED_BRAS / 4);
+        const int lo = max(a.bra_seg[kh.mol] - cb, w0);
+        const int hi = min(min(a.bra_seg[kh.mol + 1] - cb, w0 + JRED_BRAS / 4), nbt);
+        const int hnd = a.same ? min(hi, j - cb) : hi;   // non-diagonal bras
         const bool split = a.thr32 > 0.0;
-        const double* bb = a.bra_bound;
-        // covering bras [r0, r1) of this range (est falls along the bras)
         int r0 = lo, r1 = lo;
         if (w == 0) {
-            r1 = first_true(lo, hnd, [&](int i) {
-                const double est = bb[i] * qk;
+            r1 = first_true(lo, hnd, [&](int t) {
+                const double est = s_bound[t] * qk;
                 return !(est > a.thr) || (split && est < a.thr32);
             });
         } else if (split) {
-            r0 = first_true(lo, hnd, [&](int i) { return bb[i] * qk < a.thr32; });
-            r1 = first_true(r0, hnd, [&](int i) { return !(bb[i] * qk > a.thr); });
+            r0 = first_true(lo, hnd, [&](int t) { return s_bound[t] * qk < a.thr32; });
+            r1 = first_true(r0, hnd, [&](int t) { return !(s_bound[t] * qk > a.thr); });
         }
-        auto add = [&](int i) {
-            const long long k = j - b.j0[i];
-            if (k < 0 || k >= b.cnt[i]) return;   // never taken: the predicates match the counts
-            const long long q = b.qs[i] + k;
-            for (int tt = 0; tt < ntiles; ++tt) sum += jkbuf[(size_t)(tt * ncd + cd) * nq + q];
+        const double* col = jkbuf + (size_t)cd * nq + j;
+        auto add = [&](int t) {
+            const int k = j - s_j0[t];
+            if (k < 0 || k >= s_cnt[t]) return;   // never taken: the predicates match the counts
+            const double* p = col + s_base[t];
+            for (int tt = 0; tt < ntiles; ++tt) sum += p[(size_t)tt * ncd * nq];
         };
-#pragma unroll 4
-        for (int i = r0; i < r1; ++i) add(i);
-        if (w == 0 && a.same && j >= lo && j < hi) add(j);
+#pragma unroll 8
+        for (int t = r0; t < r1; ++t) add(t);
+        if (w == 0 && a.same && j - cb >= lo && j - cb < hi) add(j - cb);
     }
     if (wv > 0) part[wv - 1][lane] = sum;
     __syncthreads();
@@ -185,6 +199,29 @@ __global__ void __launch_bounds__(256) jket_reduce_kernel(const ListArgs a, cons
             atomic_addd(&J[mi.base + (size_t)(o3 + c) * mi.n + o4 + d], sum);
         }
     }
+}
+
+__global__ void __launch_bounds__(256) jket_reduce_kernel(const ListArgs a, const ListBras b, int w,
+                                                          const double* jkbuf, long long nq, int ntiles, int ncd,
+                                                          int n4, const MolInfo* mols, double* J)
+{
+    jket_reduce_tile(a, b, w, jkbuf, nq, ntiles, ncd, n4, mols, J, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// All lists in one launch: block -> list by the table's first-block offsets
+// (a handful of entries, scanned by every thread alike).
+__global__ void __launch_bounds__(256) jket_reduce_all_kernel(const JredList* lists, int nlist, const MolInfo* mols,
+                                                              double* J)
+{
+    int c = 0;
+    while (c + 1 < nlist && lists[c + 1].block0 <= (long long)blockIdx.x) ++c;
+    const JredList& L = lists[c];
+    const long long r = blockIdx.x - L.block0;
+    const int bxk = (int)(r % L.bx);
+    const long long rest = r / L.bx;
+    const int byb = (int)(rest % L.by);
+    const int cd = (int)(rest / L.by);
+    jket_reduce_tile(L.a, L.b, L.w, L.jkbuf, L.nq, L.ntiles, L.ncd, L.n4, mols, J, bxk, byb, cd);
 }
 
 static inline unsigned blocks_of(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -217,6 +254,31 @@ void list_items_fill(const ListArgs& a, const ListBras& b, long long nitems, lon
     if (nitems > 0)
         hipLaunchKernelGGL(list_items_fill_kernel, dim3(blocks_of(nitems, 256)), dim3(256), 0, st, a, b, nitems, run,
                            items);
+}
+
+JredList jred_list(const ListArgs& a, const ListBras& b, int w, const double* jkbuf, long long nq, int ntiles, int ncd,
+                   int n4)
+{
+    JredList L;
+    L.a = a;
+    L.b = b;
+    L.jkbuf = jkbuf;
+    L.nq = nq;
+    L.w = w;
+    L.ntiles = ntiles;
+    L.ncd = ncd;
+    L.n4 = n4;
+    L.bx = (int)blocks_of(a.nk, 64);
+    L.by = (int)blocks_of(a.nb, JRED_BRAS);
+    L.block0 = 0;
+    return L;
+}
+
+void jket_reduce_all(const JredList* lists_dev, int nlist, long long nblocks, const MolInfo* mols, double* J,
+                     hipStream_t st)
+{
+    if (nlist > 0 && nblocks > 0)
+        hipLaunchKernelGGL(jket_reduce_all_kernel, dim3((unsigned)nblocks), dim3(256), 0, st, lists_dev, nlist, mols, J);
 }
 
 void jket_reduce(const ListArgs& a, const ListBras& b, int w, const double* jkbuf, long long nq, int ntiles, int ncd,

@@ -196,6 +196,10 @@ struct pq_basis {
     // per pair class: pair ids sorted by (molecule, bound desc), molecule
     // segment starts, cumulative primitive-pair counts (device)
     DevBuf d_sorted[6], d_seg[6], d_npcum[6], d_sbound[6], d_total;
+    // every tiled list's ket J reduction, launched once per J/K call
+    DevBuf d_jred;
+    int njred = 0;
+    long long jred_blocks = 0;
     int nsorted[6] = {};
     DevBuf d_pairs, d_prims, d_atoms, d_cart_off, d_bound, d_tmp_list, d_spher, d_mols;
     int any_spher = 0;
@@ -425,7 +429,7 @@ extern "C" void pq_basis_destroy(pq_basis* b)
     }
     b->d_pairs.release(); b->d_prims.release(); b->d_atoms.release();
     b->d_cart_off.release(); b->d_bound.release(); b->d_tmp_list.release(); b->d_spher.release();
-    b->d_mols.release(); b->d_total.release();
+    b->d_mols.release(); b->d_total.release(); b->d_jred.release();
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->ev_jk[0]) {
         for (int i = 0; i < 2 * ERI_NCLASS; ++i)
@@ -642,10 +646,11 @@ static ListBras list_bras(EriClassStore& s)
 }
 
 // J/K work items of one list on the device: runs of equal bra pair cut into
-// pieces of at most jk_run quartets.  Automatic length: shrink for classes
-// with few bra pairs so that every class still launches ~2048 blocks.  The
-// block size follows the mean item length (64..256).  Synchronises `st` to
-// learn the item count.
+// pieces of at most jk_run quartets.  Automatic length: every item stages and
+// flushes 2 (TA + TB) rows of n elements, so its ERI values (run * ncomp)
+// should outweigh that four times over; capped so that the class keeps at
+// least 512 items, within 64..2048.  The block size follows the mean item
+// length (64..256).  Synchronises `st` to learn the item count.
 static int build_items(pq_basis* b, int c, int w, hipStream_t st)
 {
     EriClassStore& s = w ? b->store32[c] : b->store64[c];
@@ -653,8 +658,10 @@ static int build_items(pq_basis* b, int c, int w, hipStream_t st)
     if (!s.nq) return 0;
     long long run = s.jk_run;
     if (run <= 0) {
-        run = s.nq / 2048;
-        run = run < 64 ? 64 : (run > 1024 ? 1024 : run);
+        const long long rows = jk_tile_table[c][3] / 2;
+        run = (4LL * rows * b->nmax + s.ncomp - 1) / s.ncomp;
+        run = std::min(run, std::max(64LL, (long long)s.nq / 512));
+        run = std::max(64LL, std::min(2048LL, (run + 63) / 64 * 64));
     }
     const ListArgs a = list_args(b, c);
     const ListBras lb = list_bras(s);
@@ -746,6 +753,29 @@ extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold,
             if (build_items(b, c, w, st)) return -1;
         }
     }
+    // table of the ket J reductions of every tiled list
+    std::vector<JredList> jr;
+    long long blocks = 0;
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        if (!jk_tile_fits(b, c)) continue;
+        const int n3 = ncart_l(eri_class_table[c][2]), n4 = ncart_l(eri_class_table[c][3]);
+        for (int w = 0; w < 2; ++w) {
+            EriClassStore& s = w ? b->store32[c] : b->store64[c];
+            if (!s.nq) continue;
+            JredList L = jred_list(list_args(b, c), list_bras(s), w, (const double*)s.jkbuf.p, s.nq,
+                                   jk_tile_table[c][2], n3 * n4, n4);
+            L.block0 = blocks;
+            blocks += (long long)L.bx * L.by * L.ncd;
+            jr.push_back(L);
+        }
+    }
+    if (blocks > 0x7fffffffLL) return fail("pq_eri_plan: ket J reduction grid too large");
+    b->njred = (int)jr.size();
+    b->jred_blocks = blocks;
+    if (b->d_jred.ensure(sizeof(JredList) * std::max<size_t>(1, jr.size()))) return -1;
+    if (!jr.empty())
+        HIPCHK(hipMemcpyAsync(b->d_jred.p, jr.data(), sizeof(JredList) * jr.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -853,15 +883,17 @@ extern "C" int pq_eri_stats(const pq_basis* b, long long* stats, int nstats)
 extern "C" int pq_eri_nclass(void) { return ERI_NCLASS; }
 
 // One stored class (w = 0: FP64 list, 1: FP32 list) into J', K'.
+// part: 1 = tiled digestion kernel, 2 = ket J reduction, 3 = both.
 static int jk_launch_class(pq_basis* b, int c, int w, const double* Dj, const double* Dk, double* J, double* K,
-                           hipStream_t st)
+                           hipStream_t st, int part = 3)
 {
     EriClassStore& s = w ? b->store32[c] : b->store64[c];
     if (!s.nq) return 0;
     const MolInfo* dm = (const MolInfo*)b->d_mols.p;
     if (jk_tile_fits(b, c)) {
         double* jb = (double*)s.jkbuf.p;
-        if (w) {
+        if (!(part & 1)) {
+        } else if (w) {
             JkTileArgs<float> a{(const JkItem*)s.items.p, (const unsigned*)s.kets.p, dm, (const float*)s.vals.p,
                                 Dj, Dk, J, K, jb, s.nq, b->nmax, s.nitems, jk_grid_of(s)};
             jktile_launch<float>(c, a, s.jk_block, st);
@@ -871,7 +903,8 @@ static int jk_launch_class(pq_basis* b, int c, int w, const double* Dj, const do
             jktile_launch<double>(c, a, s.jk_block, st);
         }
         const int n3 = ncart_l(eri_class_table[c][2]), n4 = ncart_l(eri_class_table[c][3]);
-        jket_reduce(list_args(b, c), list_bras(s), w, jb, s.nq, jk_tile_table[c][2], n3 * n4, n4, dm, J, st);
+        if (part & 2)
+            jket_reduce(list_args(b, c), list_bras(s), w, jb, s.nq, jk_tile_table[c][2], n3 * n4, n4, dm, J, st);
     } else if (w) {
         JkArgs<float> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p, dm,
                         (const float*)s.vals.p, Dj, Dk, J, K, s.nq, b->nmax};
@@ -885,10 +918,11 @@ static int jk_launch_class(pq_basis* b, int c, int w, const double* Dj, const do
     return 0;
 }
 
-// Zero J', K'; digest every stored class; symmetrise.  With jk_nstreams > 1
-// the class launches fork from `st` onto side streams (largest stored bytes
-// first, each to the least-loaded stream) and join back before the final
-// kernel, so that small high-l classes overlap the large ones.
+// Zero J', K'; digest every stored class; add the ket J values of all tiled
+// lists in one reduction launch; symmetrise.  With jk_nstreams > 1 the class
+// launches fork from `st` onto side streams (largest stored bytes first, each
+// to the least-loaded stream) and join back before the reduction, so that
+// small high-l classes overlap the large ones.
 static int jk_launch_all(pq_basis* b, const double* Dj, const double* Dk, double* J, double* K, hipStream_t st)
 {
     const long long tot = b->mat_total;
@@ -917,13 +951,14 @@ static int jk_launch_all(pq_basis* b, const double* Dj, const double* Dk, double
         for (int t = 1; t < ns; ++t)
             if (load[t] < load[k]) k = t;
         load[k] += o.first;
-        if (jk_launch_class(b, o.second / 2, o.second % 2, Dj, Dk, J, K, ns > 1 ? b->jk_side[k] : st)) return -1;
+        if (jk_launch_class(b, o.second / 2, o.second % 2, Dj, Dk, J, K, ns > 1 ? b->jk_side[k] : st, 1)) return -1;
     }
     if (ns > 1)
         for (int k = 0; k < ns; ++k) {
             HIPCHK(hipEventRecord(b->ev_join[k], b->jk_side[k]));
             HIPCHK(hipStreamWaitEvent(st, b->ev_join[k], 0));
         }
+    jket_reduce_all((const JredList*)b->d_jred.p, b->njred, b->jred_blocks, dm, J, st);
     int gx = (int)((b->nmax * (long long)b->nmax + 255) / 256);
     gx = gx < 1 ? 1 : (gx > 64 ? 64 : gx);
     hipLaunchKernelGGL(jk_finish_kernel, dim3(gx, b->nmol), dim3(256), 0, st, J, K, dm);
@@ -956,17 +991,20 @@ extern "C" int pq_jk_set_params(pq_basis* b, int cls, int run_len, int grid, voi
     return 0;
 }
 
-// Digest one class into J', K' (no zero fill, no symmetrisation): timing unit
-// for the autotuner.  Returns the number of work items in *nitems.
-extern "C" int pq_jk_class(pq_basis* b, int cls, const double* Dj, const double* Dk, double* J, double* K,
+// Digest one FP64 class into J', K' (no zero fill, no symmetrisation): timing
+// unit for the autotuner and the roofline.  part: 1 = tiled digestion kernel
+// only, 2 = ket J reduction only (reads what part 1 stored), 3 = both.
+// Returns the number of work items in *nitems.
+extern "C" int pq_jk_class(pq_basis* b, int cls, int part, const double* Dj, const double* Dk, double* J, double* K,
                            int* nitems, void* stream)
 {
     if (!b || cls < 0 || cls >= ERI_NCLASS || !Dj || !Dk || !J || !K) return fail("pq_jk_class: bad argument");
+    if (part < 1 || part > 3) return fail("pq_jk_class: part must be 1, 2 or 3");
     hipStream_t st = (hipStream_t)stream;
     if (!jk_tile_fits(b, cls)) return fail("pq_jk_class: LDS rows too large for this basis");
     EriClassStore& s = b->store64[cls];
     if (nitems) *nitems = s.nitems;
-    return jk_launch_class(b, cls, 0, Dj, Dk, J, K, st);
+    return jk_launch_class(b, cls, 0, Dj, Dk, J, K, st, part);
 }
 
 extern "C" int pq_jk(pq_basis* b, const double* Dj, const double* Dk, double* J, double* K, void* stream)

@@ -114,8 +114,8 @@ class Engine:
         st = np.zeros(nc * 6, dtype=np.int64)
         check(_lib.lib().pq_eri_stats(self.h, st.ctypes.data_as(_lib.P_ll), nc * 6), "pq_eri_stats")
         st = st.reshape(nc, 6)
-        return [dict(cls="%04d" % r[5], nq64=int(r[0]), nq32=int(r[1]), ncomp=int(r[2]),
-                     nprim64=int(r[3]), nprim32=int(r[4])) for r in st]
+        return [dict(index=i, cls="%04d" % r[5], nq64=int(r[0]), nq32=int(r[1]), ncomp=int(r[2]),
+                     nprim64=int(r[3]), nprim32=int(r[4])) for i, r in enumerate(st)]
 
     def jk(self, Dj, Dk):
         """J (from Dj) and K (from Dk, with the reference's minus sign), final basis."""
