@@ -165,6 +165,7 @@ __global__ void __launch_bounds__(256) jk_tile_kernel(const JkTileArgs<real> a)
     const int tid = threadIdx.x, nt = blockDim.x;
     const size_t nq = a.nq;
     const real* __restrict__ vals = a.vals;
+    double* __restrict__ jkbuf = a.jkbuf;
     for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
         const JkItem it = a.items[item];
         const MolInfo mi = a.mols[it.mol];
@@ -214,6 +215,18 @@ __global__ void __launch_bounds__(256) jk_tile_kernel(const JkTileArgs<real> a)
             }
 #pragma unroll
             for (int c = 0; c < N3; ++c) {
+                // all loads of this ket row first (values and Dj elements), so
+                // that they are in flight together
+                double vv[N4][TA][TB], dj[N4];
+#pragma unroll
+                for (int d = 0; d < N4; ++d) {
+                    dj[d] = Dj[(size_t)(o3 + c) * n + o4 + d];
+#pragma unroll
+                    for (int x = 0; x < TA; ++x)
+#pragma unroll
+                        for (int y = 0; y < TB; ++y)
+                            vv[d][x][y] = (double)vals[(size_t)((((a0 + x) * N2 + b0 + y) * N3 + c) * N4 + d) * nq + q];
+                }
                 double d13[TA], d23[TB], k13[TA], k23[TB];
 #pragma unroll
                 for (int x = 0; x < TA; ++x) {
@@ -227,22 +240,20 @@ __global__ void __launch_bounds__(256) jk_tile_kernel(const JkTileArgs<real> a)
                 }
 #pragma unroll
                 for (int d = 0; d < N4; ++d) {
-                    const double djcd = Dj[(size_t)(o3 + c) * n + o4 + d];
                     double jcd = 0.0;
 #pragma unroll
                     for (int x = 0; x < TA; ++x)
 #pragma unroll
                         for (int y = 0; y < TB; ++y) {
-                            const int comp = (((a0 + x) * N2 + b0 + y) * N3 + c) * N4 + d;
-                            const double v = sc * (double)vals[(size_t)comp * nq + q];
-                            j12[x][y] += djcd * v;
+                            const double v = sc * vv[d][x][y];
+                            j12[x][y] += dj[d] * v;
                             jcd += d12[x][y] * v;
                             k13[x] += d24[y][d] * v;
                             k14[x][d] += d23[y] * v;
                             k23[y] += d14[x][d] * v;
                             k24[y][d] += d13[x] * v;
                         }
-                    a.jkbuf[(size_t)(tile * N3 * N4 + c * N4 + d) * nq + q] = 4.0 * jcd;
+                    jkbuf[(size_t)(tile * N3 * N4 + c * N4 + d) * nq + q] = 4.0 * jcd;
                 }
 #pragma unroll
                 for (int x = 0; x < TA; ++x) atomicAdd(&krows[x * n + o3 + c], 2.0 * k13[x]);
