@@ -25,8 +25,10 @@ class DIIS:
 
     @staticmethod
     def residual(S, D, F, X):
-        """Xt (S D F - F D S) X, zero-padded to N x N (diis.py:48-54)."""
-        r = S @ D @ F - F @ D @ S
+        """Xt (S D F - F D S) X, zero-padded to N x N (diis.py:48-54); with
+        S, D, F symmetric, F D S is the transpose of S D F."""
+        A = S @ D @ F
+        r = A - A.T
         t = X.T @ r @ X
         out = r * 0.0
         n = t.shape[0]
@@ -38,15 +40,27 @@ class DIIS:
         return float((a * b).sum())
 
     @staticmethod
+    def _dots(a, bs):
+        """<a, b> for every b, with one device-to-host transfer for tensors."""
+        if hasattr(a, "device") and getattr(a.device, "type", "cpu") != "cpu":
+            import torch
+            return torch.stack([(a * b).sum() for b in bs]).tolist()
+        return [float((a * b).sum()) for b in bs]
+
+    @staticmethod
     def _cond(m):
         ev = np.abs(np.linalg.eigvals(m))
         return np.inf if ev.min() == 0.0 else ev.max() / ev.min()
 
     def extrapolate(self, res, F):
-        self.error = float(res.max())
+        if hasattr(res, "device") and getattr(res.device, "type", "cpu") != "cpu":
+            import torch
+            self.error, amax = torch.stack([res.max(), res.abs().max()]).tolist()
+        else:
+            self.error, amax = float(res.max()), float(abs(res).max())
         if abs(self.error) >= abs(self.threshold):
             return F
-        if float(abs(res).max()) < 1e-14:
+        if amax < 1e-14:
             return F          # converged: nothing to extrapolate
         self.residuals.append(res)
         self.focks.append(F)
@@ -57,11 +71,10 @@ class DIIS:
         B[-1, -1] = 0.0
         if self.matrix is None:
             for i, r1 in enumerate(self.residuals):
-                for j, r2 in enumerate(self.residuals):
-                    B[i, j] = self._dot(r1, r2)
+                B[i, :size - 1] = self._dots(r1, self.residuals)
         else:
             B[:size - 1, :size - 1] = self.matrix
-            new = [self._dot(self.residuals[-1], r) for r in self.residuals]
+            new = self._dots(self.residuals[-1], self.residuals)
             B[:len(new), size - 2] = new
             B[size - 2, :len(new)] = new
         self.matrix = B

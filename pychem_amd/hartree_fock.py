@@ -2,10 +2,20 @@
 (Methods/hartree_fock.py).
 
 Integrals, Schwarz screening, ERI storage and every J/K build run on the GPU
-(``Engine``); the FP64 Fock diagonalisation, DIIS and energy sums run on the
-host, as in the reference, on torch CPU tensors with a bounded thread pool
-(one BLAS/LAPACK runtime, no oversubscription).  RHF is the UHF code path with
-equal alpha/beta occupations, exactly as the reference codes it
+(``Engine``).  Two SCF loops share the set-up and the guess:
+
+* ``loop="device"`` (default): the whole iteration stays in HBM.  The new
+  density is the projector onto the occupied subspace of X^T F X, obtained by
+  trace-correcting purification (matrix products only) instead of a
+  diagonalisation; DIIS residuals, extrapolation and energies are device
+  tensors, and only the small DIIS system and a few scalars cross to the host.
+  The orbitals are diagonalised once, at convergence.
+* ``loop="host"``: the reference's sequence on torch CPU tensors with a
+  bounded thread pool: diagonalise X^T F X every iteration
+  (Methods/hartree_fock.py:168-184).
+
+Both converge to the same fixed point.  RHF is the UHF code path with equal
+alpha/beta occupations, exactly as the reference codes it
 (Methods/hartree_fock.py:23-154); when alpha and beta coincide the shared
 matrices are computed once.
 
@@ -195,6 +205,37 @@ def calculate_energy(molecule, state):
                           + float(torch.sum(state.Beta.Density * state.Beta.Fock)))
 
 
+def purify(Fo, nocc, tol=1e-11, maxit=200):
+    """Projector onto the ``nocc`` lowest eigenvectors of the symmetric
+    matrix ``Fo`` by trace-correcting purification (TC2: Niklasson, Phys. Rev.
+    B 66, 155115 (2002)): start from (e_max - Fo)/(e_max - e_min) with
+    Gershgorin bounds, then X <- X^2 or 2X - X^2, whichever brings tr X closer
+    to nocc.  Stops once tr(X - X^2) < tol and takes one more step.  Works on
+    device tensors; one scalar transfer every four steps."""
+    m = Fo.shape[0]
+    eye = torch.eye(m, dtype=Fo.dtype, device=Fo.device)
+    if nocc <= 0:
+        return Fo * 0.0
+    if nocc >= m:
+        return eye.clone()
+    diag = Fo.diagonal()
+    rad = Fo.abs().sum(1) - diag.abs()
+    emin, emax = torch.stack([(diag - rad).min(), (diag + rad).max()]).tolist()
+    X = (emax * eye - Fo) / (emax - emin)
+    last = False
+    for it in range(maxit):
+        X2 = X @ X
+        t1, t2 = X.trace(), X2.trace()
+        up = (t2 - nocc).abs() < (2.0 * t1 - t2 - nocc).abs()
+        X = torch.where(up, X2, 2.0 * X - X2)
+        if last:
+            break
+        if it % 4 == 3:
+            if float(t1 - t2) < tol:
+                last = True
+    return 0.5 * (X + X.T)
+
+
 def _as_host(M):
     if isinstance(M, torch.Tensor):
         return M.to(HOST, F64)
@@ -203,23 +244,27 @@ def _as_host(M):
 
 def do(molecule, conv=structures.ENERGY_CONVERGENCE, maxiter=25, use_diis=True, diis_start=1,
        guess="CORE", threshold=structures.INTEGRAL_THRESHOLD, fp32_threshold=0.0, initial_run=True,
-       verbose=False, timing=False, diis_mode="pulay"):
+       verbose=False, timing=False, diis_mode="pulay", loop="device"):
     """SCF for the ground state (Methods/hartree_fock.py:23-154).
 
     ``guess`` is "CORE" (the reference default), "SAD", or an (N x N) density
     matrix per spin given as a tuple (Da, Db), or a single density used for
     both.  ``diis_mode`` is "pulay" (default) or "reference" (the reference's
-    gated DIIS).  Returns a dict with the energy, iteration count,
+    gated DIIS).  ``loop`` is "device" (default: the iteration stays on the
+    GPU, densities by purification) or "host" (the reference's per-iteration
+    diagonalisation on the host).  Returns a dict with the energy, iteration count,
     convergence flag, MOs, orbital energies, densities (numpy) and per-stage
     timings.
     """
+    if loop not in ("device", "host"):
+        raise ValueError("loop must be 'device' or 'host'")
     with host_threads(molecule.NOrbitals):
         return _do(molecule, conv, maxiter, use_diis, diis_start, guess, threshold, fp32_threshold,
-                   initial_run, verbose, timing, diis_mode)
+                   initial_run, verbose, timing, diis_mode, loop)
 
 
 def _do(molecule, conv, maxiter, use_diis, diis_start, guess, threshold, fp32_threshold, initial_run,
-        verbose, timing, diis_mode):
+        verbose, timing, diis_mode, loop="host"):
     t0 = time.perf_counter()
     timings = {}
     if timing:
@@ -259,6 +304,8 @@ def _do(molecule, conv, maxiter, use_diis, diis_start, guess, threshold, fp32_th
         state.Alpha.Density = _as_host(Da)
         state.Beta.Density = state.Alpha.Density if Db is Da else _as_host(Db)
     calculate_energy(molecule, state)
+    if loop == "device":
+        return _scf_device(molecule, state, conv, maxiter, use_diis, diis_start, verbose, timing, t0, timings)
     dE = 1.0
     it = 0
     history = []
@@ -304,6 +351,10 @@ def _do(molecule, conv, maxiter, use_diis, diis_start, guess, threshold, fp32_th
         state.TotalEnergy = state.Energy + molecule.NuclearRepulsion
     molecule.State = state
 
+    return _result(molecule, state, it, dE, conv, history, timings)
+
+
+def _result(molecule, state, it, dE, conv, history, timings):
     def npy(x):
         return None if x is None else x.numpy()
     return dict(energy=state.TotalEnergy, electronic=state.Energy, iterations=it,
@@ -311,3 +362,83 @@ def _do(molecule, conv, maxiter, use_diis, diis_start, guess, threshold, fp32_th
                 ea=npy(state.Alpha.Energies), eb=npy(state.Beta.Energies), Da=npy(state.Alpha.Density),
                 Db=npy(state.Beta.Density), history=history, timings=timings,
                 nuclear=molecule.NuclearRepulsion)
+
+
+# The device loop stops on the reference's |dE| <= conv only when the orbital
+# gradient has settled too (max |X^T (S D F - F D S) X| <= sqrt(conv)): an
+# oscillating SCF can pass through a small energy change far from convergence.
+
+
+def _scf_device(molecule, state, conv, maxiter, use_diis, diis_start, verbose, timing, t0, timings):
+    """The SCF iteration on device tensors (same sequence as the host loop:
+    J/K, Fock, DIIS extrapolation, new density, energy from the new density
+    and the extrapolated Fock), with the density from ``purify``."""
+    eng = get_engine(molecule)
+    dev = eng.device
+    H = molecule.Core.to(dev)
+    S = molecule.Overlap.to(dev)
+    X = molecule.X.to(dev)
+    Xt = X.T.contiguous()
+    na, nb = state.NAlpha, state.NBeta
+    Da = state.Alpha.Density.to(dev)
+    shared = state.Beta.Density is state.Alpha.Density or torch.equal(state.Beta.Density, state.Alpha.Density)
+    restricted = na == nb and shared
+    Db = Da if restricted else state.Beta.Density.to(dev)
+    E = state.Energy
+    dE = 1.0
+    it = 0
+    history = []
+    t2 = time.perf_counter()
+    Fa = Fb = None
+    while conv < abs(dE):
+        it += 1
+        if it > maxiter:
+            break
+        Dt = Da + Db
+        J, Ka = eng.jk(Dt, Da)
+        _record_jk_time(eng)
+        Fa = H + J + Ka
+        if restricted:
+            Fb = Fa
+        else:
+            _, Kb = eng.jk(Dt, Db)
+            _record_jk_time(eng)
+            Fb = H + J + Kb
+        ra = DIIS.residual(S, Da, Fa, X)
+        rb = ra if restricted else DIIS.residual(S, Db, Fb, X)
+        grad = torch.maximum(ra.abs().max(), rb.abs().max())
+        if use_diis and it > diis_start:
+            Fa = state.AlphaDIIS.extrapolate(ra, Fa)
+            Fb = Fa if restricted else state.BetaDIIS.extrapolate(rb, Fb)
+        Da = X @ purify(Xt @ Fa @ X, na) @ Xt
+        Db = Da if restricted else X @ purify(Xt @ Fb @ X, nb) @ Xt
+        old = E
+        E, gmax = torch.stack([0.5 * (((Da + Db) * H).sum() + (Da * Fa).sum() + (Db * Fb).sum()), grad]).tolist()
+        dE = E - old
+        if abs(dE) <= conv and gmax > conv ** 0.5:
+            dE = 2.0 * conv if dE >= 0 else -2.0 * conv   # energy still, gradient not: keep iterating
+        history.append(E + molecule.NuclearRepulsion)
+        if verbose:
+            print(" Cycle: %d  Total energy: %.12f  Change: %.3e" % (it, E + molecule.NuclearRepulsion, dE))
+    timings["scf_loop"] = time.perf_counter() - t2
+    if timing:
+        timings["jk_ms"] = list(eng.jk_ms)
+    # host state as the host loop leaves it: Fock, density, sorted MOs
+    state.Energy = E
+    state.TotalEnergy = E + molecule.NuclearRepulsion
+    if Fa is not None:
+        state.Alpha.Fock = Fa.to(HOST)
+        state.Beta.Fock = state.Alpha.Fock if Fb is Fa else Fb.to(HOST)
+    state.Alpha.Density = Da.to(HOST)
+    state.Beta.Density = state.Alpha.Density if Db is Da else Db.to(HOST)
+    make_MOs(molecule, state)
+    state.Alpha.sort_occupied()
+    if state.Beta.MOs is state.Alpha.MOs:
+        state.Beta.Energies = state.Alpha.Energies
+        state.Beta.MOs = state.Alpha.MOs
+    else:
+        state.Beta.sort_occupied()
+    timings["total"] = time.perf_counter() - t0
+    molecule.State = state
+    return _result(molecule, state, it, dE, conv, history, timings)
+
