@@ -137,7 +137,8 @@ def copy_calibration(mib):
 
 
 def mp2_measure(mol, r):
-    """Time the MP2 transform + energy on the converged RHF (device events)."""
+    """Time the MP2 transform (device events) in FP64 and with split-BF16
+    products (FP32 accumulation) on the converged RHF; energies in FP64."""
     import torch
     from pychem_amd import mp2
     from pychem_amd.engine import mp2_energy
@@ -146,19 +147,51 @@ def mp2_measure(mol, r):
     C, e = r["Ca"], r["ea"]
     n = C.shape[0]
     v = n - no
-    mp2.do(mol, r)                      # warm-up (allocations, code objects)
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
-    t0.record()
-    ov = eng.mp2_ovov(C[:, :no], C[:, no:], C[:, :no], C[:, no:])
-    t1.record()
-    ecorr = mp2_energy(ov, e[:no], e[no:], e[:no], e[no:], 0)
-    torch.cuda.synchronize()
-    ms = t0.elapsed_time(t1)
     flops = 2.0 * no * n ** 4 + 2.0 * no * v * n ** 3 + 2.0 * no * no * v * n ** 2 + 2.0 * no * no * v * v * n
-    return {"mp2_correlation": ecorr, "transform_ms": ms, "flops": flops,
-            "tflops": flops / (ms * 1e-3) / 1e12, "frac_fp64_peak": flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
-            "nocc": no, "nvir": v}
+    res = {}
+    for prec in ("fp64", "bf16x3"):
+        mp2.do(mol, r, precision=prec)          # warm-up (allocations, code objects)
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record()
+        ov = eng.mp2_ovov(C[:, :no], C[:, no:], C[:, :no], C[:, no:], precision=prec)
+        t1.record()
+        ecorr = mp2_energy(ov, e[:no], e[no:], e[:no], e[no:], 0)
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1)
+        del ov
+        res[prec] = {"mp2_correlation": ecorr, "transform_ms": ms, "tflops": flops / (ms * 1e-3) / 1e12}
+    out = dict(res["fp64"], flops=flops, frac_fp64_peak=res["fp64"]["tflops"] / FP64_PEAK_TFLOPS, nocc=no, nvir=v)
+    out["bf16x3"] = dict(res["bf16x3"], energy_error_vs_fp64=res["bf16x3"]["mp2_correlation"] - res["fp64"]["mp2_correlation"],
+                         speedup_vs_fp64=res["fp64"]["transform_ms"] / res["bf16x3"]["transform_ms"])
+    return out
+
+
+def headline_c20(steps):
+    """The metric's headline pair on C20H42/6-31G* (6d, jitter 0.02 A, seed
+    20010): plain FP64 RHF from SAD vs mixed precision (FP32 ERIs below a
+    Schwarz bound of 1e-6) from the learned guess; median whole-solve time."""
+    import torch
+
+    def factory():
+        return workload("c20h42")[0]
+    out = {}
+    for name, kw in (("plain_fp64_sad", dict(guess_kind="SAD")),
+                     ("mixed_learned", dict(guess_kind="MLP", fp32_threshold=1e-6))):
+        gpu_solve(factory, timing=False, **kw)                     # warm-up
+        times, last = [], None
+        for _ in range(max(1, steps)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            last, _ = gpu_solve(factory, timing=False, **kw)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        out[name] = {"time_to_solution_s": float(np.median(times)), "iterations": last["iterations"],
+                     "energy": last["energy"], "guess": last["timings"].get("guess_kind", kw["guess_kind"])}
+    out["speedup"] = out["plain_fp64_sad"]["time_to_solution_s"] / out["mixed_learned"]["time_to_solution_s"]
+    out["energy_error"] = out["mixed_learned"]["energy"] - out["plain_fp64_sad"]["energy"]
+    out["workload"] = "C20H42/6-31G* (6d) RHF, conv 1e-10, jitter 0.02 A, seed 20010"
+    return out
 
 
 def batch_measure(count, steps):
@@ -310,6 +343,7 @@ def main():
                   "eri_pass_ms": float(np.sum(np.clip(mixed[-1][0]["timings"]["eri_class_ms"], 0, None))),
                   "jk_ms": float(np.mean(mixed[-1][0]["timings"]["jk_ms"]))},
     }
+    out["headline_c20"] = headline_c20(args.steps)
     if not args.no_batch:
         out["batch"] = batch_measure(args.batch_size, args.steps)
     if not args.no_cpu:

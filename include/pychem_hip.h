@@ -153,16 +153,36 @@ int pq_eri_dense_slab(pq_basis* b, int l0, int lc, double* out_dev, void* stream
 int pq_gemm_f64(int ta, int tb, int M, int N, int K, double alpha, const double* A, long long lda,
                 long long sA, const double* B, long long ldb, long long sB, double beta, double* C,
                 long long ldc, long long sC, int batch, void* stream);
+/* pq_gemm_f64's product with split-BF16 matrix-core products: every FP64
+ * operand element is rounded to FP32 and split into hi = bf16(x) and
+ * lo = bf16(x - hi); a*b = hi*hi + hi*lo + lo*hi, accumulated in FP32,
+ * written as FP64 (relative error ~1e-5 per product).  No reference
+ * counterpart: the reference's MP2 transform is FP64 numpy. */
+int pq_gemm_bf16x3(int ta, int tb, int M, int N, int K, double alpha, const double* A, long long lda,
+                   long long sA, const double* B, long long ldb, long long sB, double beta, double* C,
+                   long long ldc, long long sC, int batch, void* stream);
 
 /* AO->MO transform (ia|jb) = sum C1_mi C1v_na C2_lj C2v_sb (mn|ls) from the
  * stored ERIs, by quarter transforms over slabs of lam (at most
  * max_slab_bytes of dense AO integrals at a time; <= 0 means 4 GiB).
  * C matrices are row-major over Cartesian AOs: Co1[ncart][o1] etc.
- * ovov_dev is [o1][v1][o2][v2].  Replaces the O(N^6) loops of
- * Methods/mp2.py:37-69 (one call per spin pair: aa, bb, ab). */
+ * ovov_dev is [o1][v1][o2][v2].  precision 0: FP64 matrix-core products;
+ * 1: the pq_gemm_bf16x3 products (FP32 accumulation).  Replaces the O(N^6)
+ * loops of Methods/mp2.py:37-69 (one call per spin pair: aa, bb, ab). */
 int pq_mp2_ovov(pq_basis* b, int o1, int v1, const double* Co1, const double* Cv1, int o2, int v2,
                 const double* Co2, const double* Cv2, double* ovov_dev, long long max_slab_bytes,
-                void* stream);
+                int precision, void* stream);
+
+/* Density projector of the device SCF loop: P = projector onto the nocc
+ * lowest eigenvectors of the symmetric m x m matrix F (orthonormal basis),
+ * by trace-correcting purification (X <- X^2 or 2X - X^2 from the Gershgorin-
+ * scaled F; the products by rocBLAS dgemm).  Replaces the per-cycle
+ * diagonalisation and occupation of Methods/hartree_fock.py:168-184 (same
+ * density; orbitals are diagonalised once at convergence).  work: 2 m^2 + 4
+ * doubles.  0 < nocc < m.  Stops once tr(X - X^2) < tol (one more step) or
+ * after max_steps; steps_out / idem_out (host, may be NULL) report both. */
+int pq_purify(const double* F_dev, int m, int nocc, double* P_dev, double* work_dev, double tol, int max_steps,
+              int* steps_out, double* idem_out, void* stream);
 
 /* MP2 energy sum over ovov [o1][v1][o2][v2] with D = e_i + e_j - e_a - e_b:
  * mode 0 closed shell sum v(2v - (ib|ja))/D; mode 1 same spin

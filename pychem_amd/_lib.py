@@ -53,7 +53,12 @@ SIGNATURES = {
                             c_void_p, c_longlong, c_longlong, c_double, c_void_p, c_longlong, c_longlong,
                             c_int, c_void_p]),
     "pq_mp2_ovov": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
-                            c_void_p, c_longlong, c_void_p]),
+                            c_void_p, c_longlong, c_int, c_void_p]),
+    "pq_purify": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_double, c_int, P_int,
+                          ctypes.POINTER(c_double), c_void_p]),
+    "pq_gemm_bf16x3": (c_int, [c_int, c_int, c_int, c_int, c_int, c_double, c_void_p, c_longlong, c_longlong,
+                               c_void_p, c_longlong, c_longlong, c_double, c_void_p, c_longlong, c_longlong,
+                               c_int, c_void_p]),
     "pq_mlp_linear_fwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
                                   c_void_p]),
     "pq_mlp_linear_bwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -164,17 +164,13 @@ def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_grap
     keep = (s > structures.LINEAR_DEPENDENCE) & (s < 5e2)               # [B, nm] kept columns
     X = U * torch.where(keep, s.clamp_min(1e-300) ** -0.5, torch.zeros_like(s))[:, None, :]
     X = X * valid[:, :, None]
-    big = 1e6 * torch.diag_embed((~keep).to(torch.float64))             # pushes dropped columns up
+    dropped = (~keep).to(torch.float64)                                  # [B, nm] dropped / pad columns
     nocc = torch.as_tensor([m.NAlphaElectrons for m in mols], device=dev)
-    occ = (ar[None, :] < nocc[:, None]).to(torch.float64)                # [B, nm]
     enuc = torch.as_tensor([structures_nuclear(m) for m in mols], device=dev)
     if guess == "SAD":
         D = 0.5 * _sad_padded(eng, mols)
     else:
-        M = X.transpose(1, 2) @ Hp @ X + big
-        e, C = torch.linalg.eigh(M)
-        C = X @ C
-        D = (C * occ[:, None, :]) @ C.transpose(1, 2)
+        D = _density(X, Hp, dropped, nocc)
     if tune is not None:
         from . import autotune
         eng.Dj = eng.to_packed(2.0 * D).contiguous()
@@ -209,7 +205,8 @@ def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_grap
             eng.jk_direct()
         F = Hp + eng.to_padded(eng.J) + eng.to_padded(eng.K)
         # Pulay DIIS on X^T (S D F - F D S) X, batched
-        R = X.transpose(1, 2) @ (Sp @ D @ F - F @ D @ Sp) @ X
+        A = Sp @ D @ F
+        R = X.transpose(1, 2) @ (A - A.transpose(1, 2)) @ X
         Fh.append(F)
         Rh.append(R)
         if len(Fh) > diis_size:
@@ -231,10 +228,7 @@ def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_grap
             Fx = torch.einsum("bk,bkij->bij", c, torch.stack(Fh, 1))
         else:
             Fx = F
-        M = X.transpose(1, 2) @ Fx @ X + big
-        e, C = torch.linalg.eigh(M)
-        C = X @ C
-        Dn = (C * occ[:, None, :]) @ C.transpose(1, 2)
+        Dn = _density(X, Fx, dropped, nocc)
         En = ((Dn * (Hp + Fx)) * vmask).sum((1, 2)) + enuc
         newly = (~done) & ((En - E).abs() < conv) & (it > 1)
         D = torch.where(done[:, None, None], D, Dn)
@@ -258,6 +252,20 @@ def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_grap
         out["densities"] = [Dh[m, :eng.n[m], :eng.n[m]].copy() for m in range(B)]
     eng.close()
     return out
+
+
+def _density(X, F, dropped, nocc):
+    """D = X P X^T with P the projector onto the nocc lowest states of
+    X^T F X, by batched trace-correcting purification.  Dropped and pad
+    columns (zero rows of X^T F X) are lifted above every molecule's spectrum
+    so that they stay empty."""
+    from .hartree_fock import purify_batched
+    M = X.transpose(1, 2) @ F @ X
+    diag = M.diagonal(dim1=1, dim2=2)
+    top = (diag + M.abs().sum(2) - diag.abs()).max(1).values.clamp_min(0.0) + 1.0
+    M = M + torch.diag_embed(dropped * top[:, None])
+    P = purify_batched(M.contiguous(), nocc)
+    return X @ P @ X.transpose(1, 2)
 
 
 def structures_nuclear(mol):

@@ -385,13 +385,13 @@ def main():
             src.append("{ hipLaunchKernelGGL(eri_k%s<%s>, dim3((a.nq + ERI_BLOCK - 1) / ERI_BLOCK), dim3(ERI_BLOCK), 0, st, a); }" % (tag, rt))
             jsrc.append("void jk_run_%s_%s(const JkArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
             jsrc.append("{ hipLaunchKernelGGL((jk_kernel<%d, %d, %d, %d, %s>), dim3((a.nq + 127) / 128), dim3(128), 0, st, a); }" % (la, lb, lc, ld, rt))
-            ta, tb, nt = jk_tile(la, lb, lc, ld)
-            jsrc.append("void jktile_run_%s_%s(const JkTileArgs<%s>& a, int block, hipStream_t st)" % (tag, rn, rt))
-            jsrc.append("{ hipLaunchKernelGGL((jk_tile_kernel<%d, %d, %d, %d, %d, %d, %s>), dim3(a.grid, %d), dim3(block), "
-                        "sizeof(double) * %d * a.nmax, st, a); }" % (la, lb, lc, ld, ta, tb, rt, nt, 2 * (ta + tb)))
             jsrc.append("void scatter_run_%s_%s(const ScatterArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
             jsrc.append("{ hipLaunchKernelGGL((scatter_kernel<%d, %d, %d, %d, %s>), dim3((a.nq + 127) / 128), dim3(128), 0, st, a); }" % (la, lb, lc, ld, rt))
         _write_if_changed(os.path.join(gdir, "cls_%s.hip" % tag), "\n".join(src) + "\n")
+        ta, tb, nt = jk_tile(la, lb, lc, ld)
+        jsrc.append("void jktile_run_%s(const JkTileArgs& a, int block, hipStream_t st)" % tag)
+        jsrc.append("{ hipLaunchKernelGGL((jk_tile_kernel<%d, %d, %d, %d, %d, %d>), dim3(a.grid, %d), dim3(block), "
+                    "sizeof(double) * %d * a.nmax, st, a); }" % (la, lb, lc, ld, ta, tb, nt, 2 * (ta + tb)))
         _write_if_changed(os.path.join(gdir, "jk_%s.hip" % tag), "\n".join(jsrc) + "\n")
     onee = ["// generated by gen_eri.py -- do not edit", '#include "../eri_common.h"']
     for la, lb in PAIRS:
@@ -414,7 +414,6 @@ def main():
         for rn, rt in (("d", "double"), ("f", "float")):
             d.append("void eri_run_%s_%s(const EriArgs<%s>&, hipStream_t);" % (tag, rn, rt))
             d.append("void jk_run_%s_%s(const JkArgs<%s>&, hipStream_t);" % (tag, rn, rt))
-            d.append("void jktile_run_%s_%s(const JkTileArgs<%s>&, int, hipStream_t);" % (tag, rn, rt))
             d.append("void scatter_run_%s_%s(const ScatterArgs<%s>&, hipStream_t);" % (tag, rn, rt))
     d.append("void onee_run(int pcls, const OneeArgs& a, hipStream_t st);")
     d.append("// la, lb, lc, ld, components, bra pair class, ket pair class")
@@ -428,13 +427,13 @@ def main():
         ta, tb, nt = jk_tile(la, lb, lc, ld)
         d.append("  {%d, %d, %d, %d}," % (ta, tb, nt, 2 * (ta + tb)))
     d.append("};")
-    for rn, rt in (("d", "double"), ("f", "float")):
-        d.append("static void (*const jktile_table_%s[ERI_NCLASS])(const JkTileArgs<%s>&, int, hipStream_t) = {" % (rn, rt))
-        d.append("  " + ", ".join("jktile_run_%d%d%d%d_%s" % (t[0], t[1], t[2], t[3], rn) for t in table))
-        d.append("};")
-    d.append("template <typename real> static inline void jktile_launch(int cls, const JkTileArgs<real>& a, int block, hipStream_t st)")
-    d.append("{ if (sizeof(real) == 8) jktile_table_d[cls]((const JkTileArgs<double>&)a, block, st); "
-             "else jktile_table_f[cls]((const JkTileArgs<float>&)a, block, st); }")
+    for la, lb, lc, ld, ncomp, X, Y, nbody in table:
+        d.append("void jktile_run_%d%d%d%d(const JkTileArgs&, int, hipStream_t);" % (la, lb, lc, ld))
+    d.append("static void (*const jktile_table[ERI_NCLASS])(const JkTileArgs&, int, hipStream_t) = {")
+    d.append("  " + ", ".join("jktile_run_%d%d%d%d" % (t[0], t[1], t[2], t[3]) for t in table))
+    d.append("};")
+    d.append("static inline void jktile_launch(int cls, const JkTileArgs& a, int block, hipStream_t st)")
+    d.append("{ jktile_table[cls](a, block, st); }")
     for kname, targ in (("eri", "EriArgs"), ("jk", "JkArgs"), ("scatter", "ScatterArgs")):
         for rn, rt in (("d", "double"), ("f", "float")):
             d.append("static void (*const %s_table_%s[ERI_NCLASS])(const %s<%s>&, hipStream_t) = {" % (kname, rn, targ, rt))

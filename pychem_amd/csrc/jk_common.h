@@ -139,35 +139,42 @@ struct JkItem {
 };
 
 template <typename real>
-struct JkTileArgs {
+struct JkTileList {
     const JkItem* items;
     const unsigned* kets;    // per quartet: o3 | o4 << 15 | (s3 == s4) << 30 | (P == Q) << 31
-    const MolInfo* mols;
     const real* vals;
+    double* jkbuf;           // [ntiles * N3 * N4][nq] ket J values per quartet
+    int nq, nitems;
+};
+
+// Both stored lists of one class (FP64 values, and FP32 values below the
+// Schwarz split) in one launch: blocks take items of the FP64 list first.
+struct JkTileArgs {
+    JkTileList<double> l64;
+    JkTileList<float> l32;
+    const MolInfo* mols;
     const double* Dj;
     const double* Dk;
     double* J;
     double* K;
-    double* jkbuf;           // [ntiles * N3 * N4][nq] ket J values per quartet
-    int nq, nmax, nitems, grid;   // grid: blocks in x (<= nitems; persistent when smaller)
+    int nmax, grid;          // grid: blocks in x (<= items; persistent when smaller)
 };
 
 template <int LA, int LB, int LC, int LD, int TA, int TB, typename real>
-__global__ void __launch_bounds__(256) jk_tile_kernel(const JkTileArgs<real> a)
+__device__ __forceinline__ void jk_tile_item(const JkTileList<real>& L, const JkTileArgs& a, int item, double* lds,
+                                             int tile)
 {
     constexpr int N1 = (LA + 1) * (LA + 2) / 2, N2 = (LB + 1) * (LB + 2) / 2;
     constexpr int N3 = (LC + 1) * (LC + 2) / 2, N4 = (LD + 1) * (LD + 2) / 2;
     static_assert(N1 % TA == 0 && N2 % TB == 0, "tile must divide the bra components");
     constexpr int NTB = N2 / TB, NR = TA + TB;
-    extern __shared__ double lds[];
-    const int tile = blockIdx.y;
     const int a0 = (tile / NTB) * TA, b0 = (tile % NTB) * TB;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const size_t nq = a.nq;
-    const real* __restrict__ vals = a.vals;
-    double* __restrict__ jkbuf = a.jkbuf;
-    for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
-        const JkItem it = a.items[item];
+    const size_t nq = L.nq;
+    const real* __restrict__ vals = L.vals;
+    double* __restrict__ jkbuf = L.jkbuf;
+    {
+        const JkItem it = L.items[item];
         const MolInfo mi = a.mols[it.mol];
         const int n = mi.n;
         double* drows = lds;            // NR rows of Dk: o1+a0.., then o2+b0..
@@ -194,7 +201,7 @@ __global__ void __launch_bounds__(256) jk_tile_kernel(const JkTileArgs<real> a)
             }
         __syncthreads();
         for (int q = it.start + tid; q < it.end; q += nt) {
-            const unsigned w = a.kets[q];
+            const unsigned w = L.kets[q];
             const int o3 = w & 0x7fff, o4 = (w >> 15) & 0x7fff;
             double sc = sbra;
             if ((w >> 30) & 1u) sc *= 0.5;
@@ -285,6 +292,19 @@ __global__ void __launch_bounds__(256) jk_tile_kernel(const JkTileArgs<real> a)
             }
         }
         __syncthreads();
+    }
+}
+
+template <int LA, int LB, int LC, int LD, int TA, int TB>
+__global__ void __launch_bounds__(256) jk_tile_kernel(const JkTileArgs a)
+{
+    extern __shared__ double lds[];
+    const int total = a.l64.nitems + a.l32.nitems;
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        if (item < a.l64.nitems)
+            jk_tile_item<LA, LB, LC, LD, TA, TB, double>(a.l64, a, item, lds, blockIdx.y);
+        else
+            jk_tile_item<LA, LB, LC, LD, TA, TB, float>(a.l32, a, item - a.l64.nitems, lds, blockIdx.y);
     }
 }
 

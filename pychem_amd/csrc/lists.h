@@ -11,6 +11,8 @@
 // makes the bras that reach a given ket j a contiguous range, which is what
 // the ket J reduction walks.
 #pragma once
+#include <vector>
+
 #include "jk_common.h"
 
 struct ListArgs {
@@ -46,20 +48,24 @@ void list_fill(const ListArgs& a, const ListBras& b, long long nq, int2* quartet
 void list_items_fill(const ListArgs& a, const ListBras& b, long long nitems, long long run, JkItem* items,
                      hipStream_t st);
 // J'_Q += sum over the bras that reach ket Q of the per-quartet ket J values
-// jkbuf[(tile * ncd + cd) * nq + q]; one list per launch, or every list of a
-// table in one launch.
+// jkbuf[(tile * ncd + cd) * nq + q], over tiles of up to 64 kets x 256 bras of
+// one molecule (int4: first ket, kets, first bra, bras); one list per launch,
+// or every list of a table in one launch.
 struct JredList {
     ListArgs a;
     ListBras b;
     const double* jkbuf;
     long long nq;
-    int w, ntiles, ncd, n4;
-    int bx, by;          // blocks over kets (64 each) and bras (256 each); ncd planes
-    long long block0;    // first block of this list in the fused launch
+    int w, ntiles, ncd, n4;  // ntiles: bra component tiles of the digestion kernel
+    const int4* tiles;
+    int ntile;
+    long long block0;        // first block of this list in the fused launch (ntile * ncd blocks)
 };
+std::vector<int4> jred_tiles(const std::vector<int>& bra_seg, const std::vector<int>& ket_seg,
+                             const std::vector<double>& bra_bound, const std::vector<double>& ket_bound, int same,
+                             double thr);
 JredList jred_list(const ListArgs& a, const ListBras& b, int w, const double* jkbuf, long long nq, int ntiles, int ncd,
-                   int n4);
+                   int n4, const int4* tiles, int ntile);
 void jket_reduce_all(const JredList* lists_dev, int nlist, long long nblocks, const MolInfo* mols, double* J,
                      hipStream_t st);
-void jket_reduce(const ListArgs& a, const ListBras& b, int w, const double* jkbuf, long long nq, int ntiles, int ncd,
-                 int n4, const MolInfo* mols, double* J, hipStream_t st);
+void jket_reduce(const JredList& L, const MolInfo* mols, double* J, hipStream_t st);

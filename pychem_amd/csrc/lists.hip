@@ -2,6 +2,8 @@
 // J reduction (the list layout and its monotonicity are described in lists.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "lists.h"
 
 // First index in [lo, hi) where pred holds, for a predicate that is false
@@ -123,8 +125,8 @@ __global__ void list_items_fill_kernel(const ListArgs a, const ListBras b, long 
     items[t] = it;
 }
 
-// Ket J reduction in tiles of 64 consecutive kets (one per lane) x 256 bras
-// (blockIdx.y; a quarter per wave) x one ket component (blockIdx.z).  The
+// Ket J reduction over tiles of up to 64 consecutive kets (one per lane) x up
+// to 256 bras of one molecule (a quarter per wave) x one ket component.  The
 // tile's bra data (first quartet minus first ket, ket range, bound) is staged
 // in LDS, so that each bra costs one global load per lane: the value of the
 // lane's ket under that bra, and neighbouring lanes read neighbouring values.
@@ -137,15 +139,15 @@ __global__ void list_items_fill_kernel(const ListArgs a, const ListBras b, long 
 #define JRED_BRAS 256
 __device__ __forceinline__ void jket_reduce_tile(const ListArgs& a, const ListBras& b, int w, const double* jkbuf,
                                                  long long nq, int ntiles, int ncd, int n4, const MolInfo* mols,
-                                                 double* J, int bxk, int byb, int cd)
+                                                 double* J, int4 tl, int cd)
 {
     __shared__ long long s_base[JRED_BRAS];
     __shared__ double s_bound[JRED_BRAS];
     __shared__ int s_j0[JRED_BRAS], s_cnt[JRED_BRAS];
     __shared__ double part[3][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int cb = byb * JRED_BRAS;   // first bra of the tile
-    const int nbt = min(JRED_BRAS, a.nb - cb);
+    const int cb = tl.z;              // first bra of the tile
+    const int nbt = tl.w;             // bras in the tile (<= JRED_BRAS)
     for (int t = threadIdx.x; t < nbt; t += blockDim.x) {
         const int i = cb + t;
         s_j0[t] = b.j0[i];
@@ -154,8 +156,8 @@ __device__ __forceinline__ void jket_reduce_tile(const ListArgs& a, const ListBr
         s_bound[t] = a.bra_bound[i];
     }
     __syncthreads();
-    const int j = bxk * 64 + lane;
-    const bool valid = j < a.nk;
+    const int j = tl.x + lane;
+    const bool valid = lane < tl.y;
     double sum = 0.0;
     const int Q = valid ? a.ket[j] : 0;
     const PairHdr kh = a.pairs[Q];
@@ -201,11 +203,10 @@ __device__ __forceinline__ void jket_reduce_tile(const ListArgs& a, const ListBr
     }
 }
 
-__global__ void __launch_bounds__(256) jket_reduce_kernel(const ListArgs a, const ListBras b, int w,
-                                                          const double* jkbuf, long long nq, int ntiles, int ncd,
-                                                          int n4, const MolInfo* mols, double* J)
+__global__ void __launch_bounds__(256) jket_reduce_kernel(const JredList L, const MolInfo* mols, double* J)
 {
-    jket_reduce_tile(a, b, w, jkbuf, nq, ntiles, ncd, n4, mols, J, blockIdx.x, blockIdx.y, blockIdx.z);
+    const int t = blockIdx.x % L.ntile, cd = blockIdx.x / L.ntile;
+    jket_reduce_tile(L.a, L.b, L.w, L.jkbuf, L.nq, L.ntiles, L.ncd, L.n4, mols, J, L.tiles[t], cd);
 }
 
 // All lists in one launch: block -> list by the table's first-block offsets
@@ -217,11 +218,8 @@ __global__ void __launch_bounds__(256) jket_reduce_all_kernel(const JredList* li
     while (c + 1 < nlist && lists[c + 1].block0 <= (long long)blockIdx.x) ++c;
     const JredList& L = lists[c];
     const long long r = blockIdx.x - L.block0;
-    const int bxk = (int)(r % L.bx);
-    const long long rest = r / L.bx;
-    const int byb = (int)(rest % L.by);
-    const int cd = (int)(rest / L.by);
-    jket_reduce_tile(L.a, L.b, L.w, L.jkbuf, L.nq, L.ntiles, L.ncd, L.n4, mols, J, bxk, byb, cd);
+    const int t = (int)(r % L.ntile), cd = (int)(r / L.ntile);
+    jket_reduce_tile(L.a, L.b, L.w, L.jkbuf, L.nq, L.ntiles, L.ncd, L.n4, mols, J, L.tiles[t], cd);
 }
 
 static inline unsigned blocks_of(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -256,8 +254,36 @@ void list_items_fill(const ListArgs& a, const ListBras& b, long long nitems, lon
                            items);
 }
 
+// Tiles of one list: per molecule, runs of up to 64 kets x 256 bras; with
+// bra and ket lists the same only bras up to the tile's last ket; tiles whose
+// largest bound product (first bra x first ket, both lists sorted by
+// descending bound) cannot pass thr are dropped unless they hold a diagonal.
+std::vector<int4> jred_tiles(const std::vector<int>& bra_seg, const std::vector<int>& ket_seg,
+                             const std::vector<double>& bra_bound, const std::vector<double>& ket_bound, int same,
+                             double thr)
+{
+    std::vector<int4> out;
+    const int nmol = (int)ket_seg.size() - 1;
+    for (int m = 0; m < nmol; ++m) {
+        const int ks = ket_seg[m], ke = ket_seg[m + 1], bs = bra_seg[m], be = bra_seg[m + 1];
+        for (int k0 = ks; k0 < ke; k0 += 64) {
+            const int nk = std::min(64, ke - k0);
+            for (int b0 = bs; b0 < be; b0 += JRED_BRAS) {
+                if (same && b0 > k0 + nk - 1) break;
+                const bool diag = same && b0 <= k0 + nk - 1 && b0 + JRED_BRAS - 1 >= k0;
+                if (!diag && !(bra_bound[b0] * ket_bound[k0] > thr)) {
+                    if (same) continue;   // the diagonal tile is still ahead
+                    break;                // later bra chunks are weaker
+                }
+                out.push_back(make_int4(k0, nk, b0, std::min(JRED_BRAS, be - b0)));
+            }
+        }
+    }
+    return out;
+}
+
 JredList jred_list(const ListArgs& a, const ListBras& b, int w, const double* jkbuf, long long nq, int ntiles, int ncd,
-                   int n4)
+                   int n4, const int4* tiles, int ntile)
 {
     JredList L;
     L.a = a;
@@ -268,8 +294,8 @@ JredList jred_list(const ListArgs& a, const ListBras& b, int w, const double* jk
     L.ntiles = ntiles;
     L.ncd = ncd;
     L.n4 = n4;
-    L.bx = (int)blocks_of(a.nk, 64);
-    L.by = (int)blocks_of(a.nb, JRED_BRAS);
+    L.tiles = tiles;
+    L.ntile = ntile;
     L.block0 = 0;
     return L;
 }
@@ -281,10 +307,9 @@ void jket_reduce_all(const JredList* lists_dev, int nlist, long long nblocks, co
         hipLaunchKernelGGL(jket_reduce_all_kernel, dim3((unsigned)nblocks), dim3(256), 0, st, lists_dev, nlist, mols, J);
 }
 
-void jket_reduce(const ListArgs& a, const ListBras& b, int w, const double* jkbuf, long long nq, int ntiles, int ncd,
-                 int n4, const MolInfo* mols, double* J, hipStream_t st)
+void jket_reduce(const JredList& L, const MolInfo* mols, double* J, hipStream_t st)
 {
-    if (a.nk > 0 && a.nb > 0 && nq > 0)
-        hipLaunchKernelGGL(jket_reduce_kernel, dim3(blocks_of(a.nk, 64), blocks_of(a.nb, JRED_BRAS), ncd), dim3(256), 0,
-                           st, a, b, w, jkbuf, nq, ntiles, ncd, n4, mols, J);
+    const long long nblocks = (long long)L.ntile * L.ncd;
+    if (nblocks > 0 && L.nq > 0)
+        hipLaunchKernelGGL(jket_reduce_kernel, dim3((unsigned)nblocks), dim3(256), 0, st, L, mols, J);
 }

@@ -22,6 +22,7 @@
 
 #include "../../include/pychem_hip.h"
 #include "lists.h"
+#include "scf.h"
 #include "eri_common.h"
 #include "eri_dispatch.h"
 #include "mp2.h"
@@ -166,11 +167,11 @@ struct DevBuf {
 };
 
 struct EriClassStore {
-    DevBuf q, vals, items, kets, jkbuf;
+    DevBuf q, vals, items, kets, jkbuf, jtiles;
+    int njtile = 0;            // ket J reduction tiles
     // per bra of the class's bra list (see lists.h)
     DevBuf j0, cnt, np, qs, nit, its;
     int nitems = 0;
-    int jk_block = 64;         // J/K threads per block (from the mean item length)
     int jk_run = 0;            // J/K work-item length (0: automatic)
     int jk_grid = 0;           // J/K blocks launched (0: one per item)
     int nq = 0;
@@ -196,6 +197,8 @@ struct pq_basis {
     // per pair class: pair ids sorted by (molecule, bound desc), molecule
     // segment starts, cumulative primitive-pair counts (device)
     DevBuf d_sorted[6], d_seg[6], d_npcum[6], d_sbound[6], d_total;
+    std::vector<int> hseg[6];                // host copies: molecule segments, sorted bounds
+    std::vector<double> hsbound[6];
     // every tiled list's ket J reduction, launched once per J/K call
     DevBuf d_jred;
     int njred = 0;
@@ -420,7 +423,7 @@ extern "C" void pq_basis_destroy(pq_basis* b)
         b->store32[c].q.release(); b->store32[c].vals.release(); b->store32[c].items.release();
         for (int w = 0; w < 2; ++w) {
             EriClassStore& t = w ? b->store32[c] : b->store64[c];
-            t.kets.release(); t.jkbuf.release(); t.j0.release(); t.cnt.release(); t.np.release();
+            t.kets.release(); t.jkbuf.release(); t.jtiles.release(); t.j0.release(); t.cnt.release(); t.np.release();
             t.qs.release(); t.nit.release(); t.its.release();
         }
     }
@@ -602,6 +605,8 @@ static int upload_sorted_pairs(pq_basis* b, hipStream_t st)
             b->d_npcum[X].ensure(sizeof(long long) * npcum.size()) || b->d_sbound[X].ensure(sizeof(double) * sb.size()))
             return -1;
         HIPCHK(hipMemcpyAsync(b->d_sbound[X].p, sb.data(), sizeof(double) * sb.size(), hipMemcpyHostToDevice, st));
+        b->hseg[X] = seg;
+        b->hsbound[X] = sb;
         if (n) HIPCHK(hipMemcpyAsync(b->d_sorted[X].p, sorted.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(b->d_seg[X].p, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(b->d_npcum[X].p, npcum.data(), sizeof(long long) * npcum.size(), hipMemcpyHostToDevice,
@@ -649,8 +654,8 @@ static ListBras list_bras(EriClassStore& s)
 // pieces of at most jk_run quartets.  Automatic length: every item stages and
 // flushes 2 (TA + TB) rows of n elements, so its ERI values (run * ncomp)
 // should outweigh that four times over; capped so that the class keeps at
-// least 512 items, within 64..2048.  The block size follows the mean item
-// length (64..256).  Synchronises `st` to learn the item count.
+// least 512 items, within 64..2048.  Synchronises `st` to learn the item
+// count.
 static int build_items(pq_basis* b, int c, int w, hipStream_t st)
 {
     EriClassStore& s = w ? b->store32[c] : b->store64[c];
@@ -674,8 +679,6 @@ static int build_items(pq_basis* b, int c, int w, hipStream_t st)
     s.nitems = (int)nitems;
     if (s.items.ensure(sizeof(JkItem) * std::max(1, s.nitems))) return -1;
     list_items_fill(a, lb, nitems, run, (JkItem*)s.items.p, st);
-    const double mean = s.nitems ? (double)s.nq / s.nitems : 0.0;
-    s.jk_block = mean > 160.0 ? 256 : (mean > 80.0 ? 128 : 64);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -762,10 +765,18 @@ extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold,
         for (int w = 0; w < 2; ++w) {
             EriClassStore& s = w ? b->store32[c] : b->store64[c];
             if (!s.nq) continue;
+            const int X = eri_class_table[c][5], Y = eri_class_table[c][6];
+            const std::vector<int4> tl =
+                jred_tiles(b->hseg[X], b->hseg[Y], b->hsbound[X], b->hsbound[Y], X == Y, b->threshold);
+            s.njtile = (int)tl.size();
+            if (s.jtiles.ensure(sizeof(int4) * std::max<size_t>(1, tl.size()))) return -1;
+            if (!tl.empty())
+                HIPCHK(hipMemcpyAsync(s.jtiles.p, tl.data(), sizeof(int4) * tl.size(), hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));   // tl goes out of scope
             JredList L = jred_list(list_args(b, c), list_bras(s), w, (const double*)s.jkbuf.p, s.nq,
-                                   jk_tile_table[c][2], n3 * n4, n4);
+                                   jk_tile_table[c][2], n3 * n4, n4, (const int4*)s.jtiles.p, s.njtile);
             L.block0 = blocks;
-            blocks += (long long)L.bx * L.by * L.ncd;
+            blocks += (long long)L.ntile * L.ncd;
             jr.push_back(L);
         }
     }
@@ -883,36 +894,58 @@ extern "C" int pq_eri_stats(const pq_basis* b, long long* stats, int nstats)
 extern "C" int pq_eri_nclass(void) { return ERI_NCLASS; }
 
 // One stored class (w = 0: FP64 list, 1: FP32 list) into J', K'.
-// part: 1 = tiled digestion kernel, 2 = ket J reduction, 3 = both.
-static int jk_launch_class(pq_basis* b, int c, int w, const double* Dj, const double* Dk, double* J, double* K,
-                           hipStream_t st, int part = 3)
+// One class into J', K'.  part: 1 = tiled digestion kernel, 2 = ket J
+// reductions, 3 = both.  lists: bit 0 the FP64 list, bit 1 the FP32 list; the
+// tiled kernel digests both in one launch.
+static int jk_launch_class(pq_basis* b, int c, const double* Dj, const double* Dk, double* J, double* K,
+                           hipStream_t st, int part = 3, int lists = 3)
 {
-    EriClassStore& s = w ? b->store32[c] : b->store64[c];
-    if (!s.nq) return 0;
+    EriClassStore& s64 = b->store64[c];
+    EriClassStore& s32 = b->store32[c];
+    const bool u64 = (lists & 1) && s64.nq, u32 = (lists & 2) && s32.nq;
+    if (!u64 && !u32) return 0;
     const MolInfo* dm = (const MolInfo*)b->d_mols.p;
     if (jk_tile_fits(b, c)) {
-        double* jb = (double*)s.jkbuf.p;
-        if (!(part & 1)) {
-        } else if (w) {
-            JkTileArgs<float> a{(const JkItem*)s.items.p, (const unsigned*)s.kets.p, dm, (const float*)s.vals.p,
-                                Dj, Dk, J, K, jb, s.nq, b->nmax, s.nitems, jk_grid_of(s)};
-            jktile_launch<float>(c, a, s.jk_block, st);
-        } else {
-            JkTileArgs<double> a{(const JkItem*)s.items.p, (const unsigned*)s.kets.p, dm, (const double*)s.vals.p,
-                                 Dj, Dk, J, K, jb, s.nq, b->nmax, s.nitems, jk_grid_of(s)};
-            jktile_launch<double>(c, a, s.jk_block, st);
+        if (part & 1) {
+            JkTileArgs a;
+            a.l64 = JkTileList<double>{(const JkItem*)s64.items.p, (const unsigned*)s64.kets.p,
+                                       (const double*)s64.vals.p, (double*)s64.jkbuf.p, s64.nq, u64 ? s64.nitems : 0};
+            a.l32 = JkTileList<float>{(const JkItem*)s32.items.p, (const unsigned*)s32.kets.p,
+                                      (const float*)s32.vals.p, (double*)s32.jkbuf.p, s32.nq, u32 ? s32.nitems : 0};
+            a.mols = dm;
+            a.Dj = Dj;
+            a.Dk = Dk;
+            a.J = J;
+            a.K = K;
+            a.nmax = b->nmax;
+            const int items = a.l64.nitems + a.l32.nitems;
+            const int tuned = s64.jk_grid > 0 ? s64.jk_grid : s32.jk_grid;
+            a.grid = (tuned > 0 && tuned < items) ? tuned : items;
+            const double mean = (double)((u64 ? s64.nq : 0) + (u32 ? s32.nq : 0)) / std::max(1, items);
+            const int block = mean > 160.0 ? 256 : (mean > 80.0 ? 128 : 64);
+            if (items > 0) jktile_launch(c, a, block, st);
         }
-        const int n3 = ncart_l(eri_class_table[c][2]), n4 = ncart_l(eri_class_table[c][3]);
-        if (part & 2)
-            jket_reduce(list_args(b, c), list_bras(s), w, jb, s.nq, jk_tile_table[c][2], n3 * n4, n4, dm, J, st);
-    } else if (w) {
-        JkArgs<float> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p, dm,
-                        (const float*)s.vals.p, Dj, Dk, J, K, s.nq, b->nmax};
-        jk_launch<float>(c, a, st);
+        if (part & 2) {
+            const int n3 = ncart_l(eri_class_table[c][2]), n4 = ncart_l(eri_class_table[c][3]);
+            for (int w = 0; w < 2; ++w) {
+                EriClassStore& s = w ? s32 : s64;
+                if (!(w ? u32 : u64)) continue;
+                jket_reduce(jred_list(list_args(b, c), list_bras(s), w, (const double*)s.jkbuf.p, s.nq,
+                                      jk_tile_table[c][2], n3 * n4, n4, (const int4*)s.jtiles.p, s.njtile),
+                            dm, J, st);
+            }
+        }
     } else {
-        JkArgs<double> a{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p, dm,
-                         (const double*)s.vals.p, Dj, Dk, J, K, s.nq, b->nmax};
-        jk_launch<double>(c, a, st);
+        if (u64) {
+            JkArgs<double> a{(const int2*)s64.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p, dm,
+                             (const double*)s64.vals.p, Dj, Dk, J, K, s64.nq, b->nmax};
+            jk_launch<double>(c, a, st);
+        }
+        if (u32) {
+            JkArgs<float> a{(const int2*)s32.q.p, (const PairHdr*)b->d_pairs.p, (const int*)b->d_cart_off.p, dm,
+                            (const float*)s32.vals.p, Dj, Dk, J, K, s32.nq, b->nmax};
+            jk_launch<float>(c, a, st);
+        }
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -931,11 +964,14 @@ static int jk_launch_all(pq_basis* b, const double* Dj, const double* Dk, double
     hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, st, J, tot, K);
     const MolInfo* dm = (const MolInfo*)b->d_mols.p;
     std::vector<std::pair<double, int>> order;
-    for (int c = 0; c < ERI_NCLASS; ++c)
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        double bytes = 0.0;
         for (int w = 0; w < 2; ++w) {
             const EriClassStore& s = w ? b->store32[c] : b->store64[c];
-            if (s.nq) order.push_back({(double)s.nq * s.ncomp * (w ? 4.0 : 8.0) + 16.0 * s.nq, 2 * c + w});
+            bytes += (double)s.nq * s.ncomp * (w ? 4.0 : 8.0) + 16.0 * s.nq;
         }
+        if (bytes > 0.0) order.push_back({bytes, c});
+    }
     std::sort(order.begin(), order.end(), [](const std::pair<double, int>& u, const std::pair<double, int>& v) {
         return u.first > v.first;
     });
@@ -951,7 +987,7 @@ static int jk_launch_all(pq_basis* b, const double* Dj, const double* Dk, double
         for (int t = 1; t < ns; ++t)
             if (load[t] < load[k]) k = t;
         load[k] += o.first;
-        if (jk_launch_class(b, o.second / 2, o.second % 2, Dj, Dk, J, K, ns > 1 ? b->jk_side[k] : st, 1)) return -1;
+        if (jk_launch_class(b, o.second, Dj, Dk, J, K, ns > 1 ? b->jk_side[k] : st, 1)) return -1;
     }
     if (ns > 1)
         for (int k = 0; k < ns; ++k) {
@@ -1004,7 +1040,7 @@ extern "C" int pq_jk_class(pq_basis* b, int cls, int part, const double* Dj, con
     if (!jk_tile_fits(b, cls)) return fail("pq_jk_class: LDS rows too large for this basis");
     EriClassStore& s = b->store64[cls];
     if (nitems) *nitems = s.nitems;
-    return jk_launch_class(b, cls, 0, Dj, Dk, J, K, st, part);
+    return jk_launch_class(b, cls, Dj, Dk, J, K, st, part, 1);
 }
 
 extern "C" int pq_jk(pq_basis* b, const double* Dj, const double* Dk, double* J, double* K, void* stream)
@@ -1029,6 +1065,8 @@ extern "C" int pq_jk_graph_create(pq_basis* b, const double* Dj, const double* D
 {
     if (!b || !Dj || !Dk || !J || !K) return fail("pq_jk_graph_create: null argument");
     if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+    // side streams and their events must exist before the capture begins
+    if (b->jk_nstreams > 1 && ensure_jk_streams(b)) return -1;
     hipStream_t cs;
     HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipGraph_t g;
@@ -1036,8 +1074,9 @@ extern "C" int pq_jk_graph_create(pq_basis* b, const double* Dj, const double* D
     const int rc = jk_launch_all(b, Dj, Dk, J, K, cs);
     hipError_t e = hipStreamEndCapture(cs, &g);
     if (rc || e != hipSuccess) {
+        const std::string why = rc ? g_err : std::string(hipGetErrorString(e));
         (void)hipStreamDestroy(cs);
-        return fail(std::string("pq_jk_graph_create: capture failed: ") + hipGetErrorString(e));
+        return fail("pq_jk_graph_create: capture failed: " + why);
     }
     e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -1206,10 +1245,23 @@ extern "C" int pq_gemm_f64(int ta, int tb, int M, int N, int K, double alpha, co
     return 0;
 }
 
+extern "C" int pq_gemm_bf16x3(int ta, int tb, int M, int N, int K, double alpha, const double* A, long long lda,
+                              long long sA, const double* B, long long ldb, long long sB, double beta, double* C,
+                              long long ldc, long long sC, int batch, void* stream)
+{
+    if (!A || !B || !C) return fail("pq_gemm_bf16x3: null argument");
+    if (M < 0 || N < 0 || K <= 0 || batch < 0) return fail("pq_gemm_bf16x3: bad sizes");
+    GemmF64Args g{A, B, C, lda, ldb, ldc, sA, sB, sC, M, N, K, ta, tb, alpha, beta};
+    if (gemm_bf16x3_launch(g, batch, (hipStream_t)stream)) return fail("pq_gemm_bf16x3: launch failed");
+    return 0;
+}
+
 extern "C" int pq_mp2_ovov(pq_basis* b, int o1, int v1, const double* Co1, const double* Cv1, int o2, int v2,
                            const double* Co2, const double* Cv2, double* ovov, long long max_slab_bytes,
-                           void* stream)
+                           int precision, void* stream)
 {
+    if (precision < 0 || precision > 1) return fail("pq_mp2_ovov: precision must be 0 (FP64) or 1 (BF16x3)");
+    auto gemm = precision ? gemm_bf16x3_launch : gemm_f64_launch;
     if (!b || !Co1 || !Cv1 || !Co2 || !Cv2 || !ovov) return fail("pq_mp2_ovov: null argument");
     if (o1 <= 0 || v1 <= 0 || o2 <= 0 || v2 <= 0) return fail("pq_mp2_ovov: empty orbital space");
     if (b->nmol != 1) return fail("pq_mp2_ovov: single-molecule basis only");
@@ -1231,25 +1283,58 @@ extern "C" int pq_mp2_ovov(pq_basis* b, int o1, int v1, const double* Co1, const
         rc |= eri_dense_slab(b, (int)l0, (int)L, (double*)slab.p, st);
         // (i nu | lam sig) = sum_mu C1[mu][i] (mu nu | lam sig)
         GemmF64Args ga{Co1, (const double*)slab.p, (double*)t1.p, o1, X, X, 0, 0, 0, o1, (int)X, (int)n, 1, 0, 1.0, 0.0};
-        rc |= gemm_f64_launch(ga, 1, st);
+        rc |= gemm(ga, 1, st);
         // (i a | lam sig) = sum_nu C1v[nu][a] (i nu | lam sig), written into the full [i][a][n][n] array
         GemmF64Args gb{Cv1, (const double*)t1.p, (double*)t2.p + l0 * n, v1, Y, n * n, 0, n * Y, (long long)v1 * n * n,
                        v1, (int)Y, (int)n, 1, 0, 1.0, 0.0};
-        rc |= gemm_f64_launch(gb, o1, st);
+        rc |= gemm(gb, o1, st);
     }
     if (!rc) {
         // (i a | j sig) = sum_lam C2[lam][j] (i a | lam sig)
         GemmF64Args gc{Co2, (const double*)t2.p, (double*)t3.p, o2, n, n, 0, n * n, (long long)o2 * n, o2, (int)n,
                        (int)n, 1, 0, 1.0, 0.0};
-        rc |= gemm_f64_launch(gc, o1 * v1, st);
+        rc |= gemm(gc, o1 * v1, st);
         // (i a | j b) = sum_sig (i a | j sig) C2v[sig][b]
         GemmF64Args gd{(const double*)t3.p, Cv2, ovov, n, v2, v2, 0, 0, 0, o1 * v1 * o2, v2, (int)n, 0, 0, 1.0, 0.0};
-        rc |= gemm_f64_launch(gd, 1, st);
+        rc |= gemm(gd, 1, st);
     }
     hipError_t e = hipStreamSynchronize(st);
     slab.release(); t1.release(); t2.release(); t3.release();
     if (rc) return fail("pq_mp2_ovov: kernel launch failed");
     if (e != hipSuccess) return fail(std::string("pq_mp2_ovov: ") + hipGetErrorString(e));
+    return 0;
+}
+
+// Density projector of the SCF loop by TC2 purification (scf.hip): P =
+// projector onto the nocc lowest eigenvectors of the symmetric m x m matrix
+// F.  work: 2 m^2 + 4 doubles.  Steps run in chunks of 8 with one host read
+// of tr(X - X^2) per chunk; once below tol, one more step.
+extern "C" int pq_purify(const double* F, int m, int nocc, double* P, double* work, double tol, int max_steps,
+                         int* steps_out, double* idem_out, void* stream)
+{
+    if (!F || !P || !work) return fail("pq_purify: null argument");
+    if (m <= 0 || nocc <= 0 || nocc >= m) return fail("pq_purify: need 0 < nocc < m");
+    hipStream_t st = (hipStream_t)stream;
+    double* X = work;
+    double* X2 = work + (size_t)m * m;
+    double* t = X2 + (size_t)m * m;
+    double* bounds = t + 2;
+    if (tc2_start(F, m, X, bounds, st)) return fail("pq_purify: launch failed");
+    int done = 0;
+    double th[2] = {0.0, 0.0};
+    bool last = false;
+    while (done < max_steps) {
+        const int k = last ? 1 : std::min(8, max_steps - done);
+        if (tc2_steps(X, X2, m, nocc, k, t, st)) return fail("pq_purify: launch failed");
+        done += k;
+        if (last) break;
+        HIPCHK(hipMemcpyAsync(th, t, sizeof th, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (th[0] - th[1] < tol) last = true;
+    }
+    if (tc2_finish(X, P, m, st)) return fail("pq_purify: launch failed");
+    if (steps_out) *steps_out = done;
+    if (idem_out) *idem_out = th[0] - th[1];
     return 0;
 }
 

@@ -163,9 +163,12 @@ class Engine:
         check(_lib.lib().pq_eri_dense_slab(self.h, int(l0), int(lc), _ptr(out), _stream()), "pq_eri_dense_slab")
         return out
 
-    def mp2_ovov(self, Co1, Cv1, Co2, Cv2, max_slab_bytes=0):
+    def mp2_ovov(self, Co1, Cv1, Co2, Cv2, max_slab_bytes=0, precision="fp64"):
         """(ia|jb) [o1][v1][o2][v2] from MO coefficient blocks given over the
-        final basis (rows = AOs); Cartesian rows are formed here."""
+        final basis (rows = AOs); Cartesian rows are formed here.  precision:
+        "fp64" (FP64 matrix-core products) or "bf16x3" (split-BF16 products,
+        FP32 accumulation)."""
+        prec = {"fp64": 0, "bf16x3": 1}[precision]
         mats = []
         for C in (Co1, Cv1, Co2, Cv2):
             C = torch.as_tensor(C, dtype=torch.float64, device=self.device)
@@ -175,7 +178,7 @@ class Engine:
         o1, v1, o2, v2 = mats[0].shape[1], mats[1].shape[1], mats[2].shape[1], mats[3].shape[1]
         out = torch.empty((o1, v1, o2, v2), dtype=torch.float64, device=self.device)
         check(_lib.lib().pq_mp2_ovov(self.h, o1, v1, _ptr(mats[0]), _ptr(mats[1]), o2, v2, _ptr(mats[2]),
-                                     _ptr(mats[3]), _ptr(out), int(max_slab_bytes), _stream()), "pq_mp2_ovov")
+                                     _ptr(mats[3]), _ptr(out), int(max_slab_bytes), prec, _stream()), "pq_mp2_ovov")
         return out
 
 
@@ -190,16 +193,18 @@ def mp2_energy(ovov, eo1, ev1, eo2, ev2, mode):
     return float(out[0])
 
 
-def gemm(A, B, ta=False, tb=False, alpha=1.0):
-    """C = op(A) op(B) on the matrix cores (FP64), torch device tensors."""
+def gemm(A, B, ta=False, tb=False, alpha=1.0, precision="fp64"):
+    """C = op(A) op(B) on the matrix cores, torch device tensors (FP64 in and
+    out); precision "fp64" or "bf16x3" (pq_gemm_bf16x3)."""
+    fn = {"fp64": "pq_gemm_f64", "bf16x3": "pq_gemm_bf16x3"}[precision]
     A = A.contiguous()
     B = B.contiguous()
     M = A.shape[1] if ta else A.shape[0]
     K = A.shape[0] if ta else A.shape[1]
     N = B.shape[0] if tb else B.shape[1]
     C = torch.empty((M, N), dtype=torch.float64, device=A.device)
-    check(_lib.lib().pq_gemm_f64(int(ta), int(tb), M, N, K, float(alpha), _ptr(A), A.shape[1], 0, _ptr(B),
-                                 B.shape[1], 0, 0.0, _ptr(C), N, 0, 1, _stream()), "pq_gemm_f64")
+    check(getattr(_lib.lib(), fn)(int(ta), int(tb), M, N, K, float(alpha), _ptr(A), A.shape[1], 0, _ptr(B),
+                                  B.shape[1], 0, 0.0, _ptr(C), N, 0, 1, _stream()), fn)
     return C
 
 

@@ -205,14 +205,31 @@ def calculate_energy(molecule, state):
                           + float(torch.sum(state.Beta.Density * state.Beta.Fock)))
 
 
+_PURIFY_WORK = {}
+
+
 def purify(Fo, nocc, tol=1e-11, maxit=200):
     """Projector onto the ``nocc`` lowest eigenvectors of the symmetric
     matrix ``Fo`` by trace-correcting purification (TC2: Niklasson, Phys. Rev.
     B 66, 155115 (2002)): start from (e_max - Fo)/(e_max - e_min) with
     Gershgorin bounds, then X <- X^2 or 2X - X^2, whichever brings tr X closer
-    to nocc.  Stops once tr(X - X^2) < tol and takes one more step.  Works on
-    device tensors; one scalar transfer every four steps."""
+    to nocc.  Stops once tr(X - X^2) < tol and takes one more step.  Device
+    tensors go through the library (pq_purify: its own product, trace and
+    branch kernels); host tensors use the same steps in torch."""
     m = Fo.shape[0]
+    if Fo.device.type == "cuda" and 0 < nocc < m:
+        import ctypes
+        from . import _lib
+        from .engine import _ptr, _stream
+        key = (m, Fo.device)
+        if key not in _PURIFY_WORK:
+            _PURIFY_WORK[key] = torch.empty(2 * m * m + 4, dtype=torch.float64, device=Fo.device)
+        Fo = Fo.contiguous()
+        P = torch.empty_like(Fo)
+        steps, idem = ctypes.c_int(0), ctypes.c_double(0.0)
+        _lib.check(_lib.lib().pq_purify(_ptr(Fo), m, int(nocc), _ptr(P), _ptr(_PURIFY_WORK[key]), float(tol),
+                                        int(maxit), ctypes.byref(steps), ctypes.byref(idem), _stream()), "pq_purify")
+        return P
     eye = torch.eye(m, dtype=Fo.dtype, device=Fo.device)
     if nocc <= 0:
         return Fo * 0.0
@@ -236,6 +253,33 @@ def purify(Fo, nocc, tol=1e-11, maxit=200):
     return 0.5 * (X + X.T)
 
 
+def purify_batched(M, nocc, tol=1e-11, maxit=200):
+    """``purify`` for a batch: M [B, m, m] symmetric, nocc [B]; every matrix
+    takes its own TC2 branch each step.  Stops once tr(X - X^2) < tol for all
+    and takes one more step."""
+    B, m, _ = M.shape
+    eye = torch.eye(m, dtype=M.dtype, device=M.device).expand(B, m, m)
+    diag = M.diagonal(dim1=1, dim2=2)
+    rad = M.abs().sum(2) - diag.abs()
+    emin = (diag - rad).min(1).values
+    emax = (diag + rad).max(1).values
+    X = (emax[:, None, None] * eye - M) / (emax - emin)[:, None, None]
+    nf = nocc.to(M.dtype)
+    last = False
+    for it in range(maxit):
+        X2 = torch.bmm(X, X)
+        t1 = X.diagonal(dim1=1, dim2=2).sum(1)
+        t2 = X2.diagonal(dim1=1, dim2=2).sum(1)
+        up = (t2 - nf).abs() < (2.0 * t1 - t2 - nf).abs()
+        X = torch.where(up[:, None, None], X2, 2.0 * X - X2)
+        if last:
+            break
+        if it % 4 == 3:
+            if float((t1 - t2).max()) < tol:
+                last = True
+    return 0.5 * (X + X.transpose(1, 2))
+
+
 def _as_host(M):
     if isinstance(M, torch.Tensor):
         return M.to(HOST, F64)
@@ -244,10 +288,12 @@ def _as_host(M):
 
 def do(molecule, conv=structures.ENERGY_CONVERGENCE, maxiter=25, use_diis=True, diis_start=1,
        guess="CORE", threshold=structures.INTEGRAL_THRESHOLD, fp32_threshold=0.0, initial_run=True,
-       verbose=False, timing=False, diis_mode="pulay", loop="device"):
+       verbose=False, timing=False, diis_mode="pulay", loop="device", nets=None):
     """SCF for the ground state (Methods/hartree_fock.py:23-154).
 
-    ``guess`` is "CORE" (the reference default), "SAD", or an (N x N) density
+    ``guess`` is "CORE" (the reference default), "SAD", "MLP" (the learned
+    guess of mlp_guess, ``nets`` or the shipped networks; SAD where they do
+    not cover the molecule), or an (N x N) density
     matrix per spin given as a tuple (Da, Db), or a single density used for
     both.  ``diis_mode`` is "pulay" (default) or "reference" (the reference's
     gated DIIS).  ``loop`` is "device" (default: the iteration stays on the
@@ -260,11 +306,11 @@ def do(molecule, conv=structures.ENERGY_CONVERGENCE, maxiter=25, use_diis=True, 
         raise ValueError("loop must be 'device' or 'host'")
     with host_threads(molecule.NOrbitals):
         return _do(molecule, conv, maxiter, use_diis, diis_start, guess, threshold, fp32_threshold,
-                   initial_run, verbose, timing, diis_mode, loop)
+                   initial_run, verbose, timing, diis_mode, loop, nets)
 
 
 def _do(molecule, conv, maxiter, use_diis, diis_start, guess, threshold, fp32_threshold, initial_run,
-        verbose, timing, diis_mode, loop="host"):
+        verbose, timing, diis_mode, loop="host", nets=None):
     t0 = time.perf_counter()
     timings = {}
     if timing:
@@ -287,6 +333,13 @@ def _do(molecule, conv, maxiter, use_diis, diis_start, guess, threshold, fp32_th
     if timing:
         timings["eri_class_ms"] = get_engine(molecule).timings()[0]
 
+    if isinstance(guess, str) and guess == "MLP":
+        from . import mlp_guess
+        t3 = time.perf_counter()
+        Dm = mlp_guess.learned_guess(molecule, nets)
+        timings["guess_kind"] = "MLP" if Dm is not None else "SAD (fallback)"
+        guess = "SAD" if Dm is None else (Dm, Dm)
+        timings["guess"] = time.perf_counter() - t3
     if isinstance(guess, str) and guess == "SAD":
         from .guess import sad_density
         t3 = time.perf_counter()

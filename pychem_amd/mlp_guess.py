@@ -27,6 +27,7 @@ local frame of its atom pair and rotated back:
 import ctypes
 import itertools
 import math
+import os
 import time
 
 import numpy as np
@@ -36,7 +37,7 @@ from . import _lib, structures
 from ._lib import check
 
 ANG = 1.0 / structures.TO_BOHR
-RCUT_PAIR = 4.5          # Angstrom: pairs beyond this start from a zero block
+RCUT_PAIR = 6.0          # Angstrom: pairs beyond this start from a zero block
 RCUT_ENV = 4.0
 MU = np.linspace(0.5, 4.0, 16)
 ETA = 1.0 / (2 * 0.25 ** 2)
@@ -177,6 +178,133 @@ def local_block(D, mol, a, b, R, idx=None, ls=None):
     return np.linalg.solve(Ma.T, blk) @ np.linalg.inv(Mb)
 
 
+# ------------------------------------------------- vectorised pair data
+def _d_tables():
+    comps = structures.cart_components(2)
+    ax = [[i for i in range(3) for _ in range(c[i])] for c in comps]
+    I = np.array([a[0] for a in ax])
+    J = np.array([a[1] for a in ax])
+    index = {c: k for k, c in enumerate(comps)}
+    A = np.zeros((9, 6))
+    for k in range(3):
+        for l in range(3):
+            m = [0, 0, 0]
+            m[k] += 1
+            m[l] += 1
+            A[k * 3 + l, index[tuple(m)]] = 1.0
+    N = np.array([(math.gamma(a + 0.5) * math.gamma(b + 0.5) * math.gamma(c + 0.5)) ** -0.5 for a, b, c in comps])
+    return I, J, A, N
+
+
+_DTAB = _d_tables()
+
+
+def rot_d_batch(R):
+    """cart_rotation(2, R) for a stack R [P, 3, 3] (numpy or torch)."""
+    I, J, A, N = _DTAB
+    if isinstance(R, torch.Tensor):
+        A_, N_ = torch.as_tensor(A, dtype=R.dtype, device=R.device), torch.as_tensor(N, dtype=R.dtype, device=R.device)
+        T = R[:, I, :, None] * R[:, J, None, :]
+        P = T.reshape(-1, 6, 9) @ A_
+        return N_[:, None] * P / N_[None, :]
+    T = R[:, I, :, None] * R[:, J, None, :]
+    P = T.reshape(-1, 6, 9) @ A
+    return N[:, None] * P / N[None, :]
+
+
+def atom_rot_batch(ls, R):
+    """atom_rotation(ls, R) for a stack R [P, 3, 3]: [P, n, n]."""
+    torch_in = isinstance(R, torch.Tensor)
+    n = sum((l + 1) * (l + 2) // 2 for l in ls)
+    P = R.shape[0]
+    if torch_in:
+        M = torch.zeros((P, n, n), dtype=R.dtype, device=R.device)
+    else:
+        M = np.zeros((P, n, n))
+    o = 0
+    for l in ls:
+        k = (l + 1) * (l + 2) // 2
+        if l == 0:
+            M[:, o, o] = 1.0
+        elif l == 1:
+            M[:, o:o + 3, o:o + 3] = R
+        else:
+            M[:, o:o + 6, o:o + 6] = rot_d_batch(R)
+        o += k
+    return M
+
+
+def pair_data(mol):
+    """All atom pairs (a, b) within RCUT_PAIR, a the heavier atom, grouped
+    by type (label_a, label_b, a == b): {type: (a [P], b [P], features
+    [P, NFEAT], R [P, 3, 3])}.  Same frames and features as pair_samples,
+    computed over all pairs at once."""
+    xyz = np.array([at.Coordinates for at in mol.Atoms], dtype=float)
+    Z = np.array([at.NuclearCharge for at in mol.Atoms], dtype=float)
+    labels = [at.Label for at in mol.Atoms]
+    n = len(Z)
+    diff = xyz[None, :, :] - xyz[:, None, :]          # [a, k, 3] = x_k - x_a
+    dist = np.linalg.norm(diff, axis=2)
+    dA = dist * ANG
+    env = (dA > 1e-8) & (dA < RCUT_ENV)
+    fc = np.where(env, 0.5 * (np.cos(np.pi * dA / RCUT_ENV) + 1.0), 0.0)
+    rad = ((Z[None, :] * fc)[:, :, None] * np.exp(-ETA * (dA[:, :, None] - MU[None, None, :]) ** 2)).sum(1)
+    ii, jj = np.triu_indices(n)
+    keep = dA[ii, jj] <= RCUT_PAIR
+    ii, jj = ii[keep], jj[keep]
+    swap = Z[ii] < Z[jj]
+    a = np.where(swap, jj, ii)
+    b = np.where(swap, ii, jj)
+    P = len(a)
+    ar = np.arange(n)
+    diag = a == b
+    # first axis: towards b, or towards the nearest neighbour for (a, a)
+    dmask = np.where(ar[None, :] == a[:, None], np.inf, dist[a])
+    nn1 = np.argmin(dmask, axis=1) if n > 1 else np.zeros(P, dtype=int)
+    tgt = np.where(diag, nn1, b)
+    V = diff[a]                                          # [P, n, 3]
+    e1 = V[np.arange(P), tgt]
+    has1 = np.linalg.norm(e1, axis=1) > 0
+    e1 = e1 / np.where(has1, np.linalg.norm(e1, axis=1), 1.0)[:, None]
+    # second axis: nearest remaining atom with a component normal to e1
+    w = V - (V @ e1[:, :, None]) * e1[:, None, :]
+    wn = np.linalg.norm(w, axis=2)
+    cand = (ar[None, :] != a[:, None]) & (ar[None, :] != b[:, None]) & (wn > 1e-3)
+    cand &= ~(diag[:, None] & (ar[None, :] == nn1[:, None]))
+    dc = np.where(cand, dist[a], np.inf)
+    k2 = np.argmin(dc, axis=1)
+    has2 = np.isfinite(dc[np.arange(P), k2])
+    e2 = w[np.arange(P), k2] / np.where(has2, wn[np.arange(P), k2], 1.0)[:, None]
+    t = np.where((np.abs(e1[:, 0]) < 0.9)[:, None], np.array([1.0, 0.0, 0.0]), np.array([0.0, 1.0, 0.0]))
+    tf = t - (t * e1).sum(1, keepdims=True) * e1
+    tf = tf / np.linalg.norm(tf, axis=1, keepdims=True)
+    e2 = np.where(has2[:, None], e2, tf)
+    e3 = np.cross(e1, e2)
+    R = np.stack([e1, e2, e3], axis=1)
+    R = np.where(has1[:, None, None], R, np.eye(3)[None])
+    # local coordinates of the frame-defining atoms (nearest neighbour, then e2 atom, for (a, a))
+    loc_e2 = np.einsum("pij,pj->pi", R, V[np.arange(P), k2]) * ANG
+    loc_nn = np.einsum("pij,pj->pi", R, V[np.arange(P), nn1]) * ANG
+    loc0 = np.where(diag[:, None], np.where(has1[:, None], loc_nn, 0.0), np.where(has2[:, None], loc_e2, 0.0))
+    loc1 = np.where((diag & has1 & has2)[:, None], loc_e2, 0.0)
+    F = np.concatenate([dA[a, b][:, None], rad[a], rad[b], loc0[:, :2], loc1[:, :2]], axis=1)
+    groups = {}
+    for p in range(P):
+        groups.setdefault((labels[a[p]], labels[b[p]], bool(diag[p])), []).append(p)
+    return {key: (a[v], b[v], F[v], R[v]) for key, v in groups.items()}
+
+
+def _atom_layout(mol):
+    """Per atom: first function index and its shells' l list."""
+    off, ls = [], []
+    o = 0
+    for atom in mol.Atoms:
+        off.append(o)
+        ls.append([cg.AngularMomentum for cg in atom.Basis])
+        o += sum(cg.NAngMom for cg in atom.Basis)
+    return np.array(off), ls
+
+
 # ----------------------------------------------------------------- network
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
@@ -271,92 +399,149 @@ class MLP:
 
 
 # ----------------------------------------------------------------- data
+def local_blocks(D, mol, groups=None):
+    """{type: (features [P, NFEAT], local blocks [P, n_a n_b])} from a density
+    D (numpy, final Cartesian basis): D' = M_a(R^T)^T D_ab M_b(R^T)."""
+    groups = pair_data(mol) if groups is None else groups
+    off, ls = _atom_layout(mol)
+    out = {}
+    for key, (a, b, F, R) in groups.items():
+        la, lb = ls[a[0]], ls[b[0]]
+        na, nb = sum((l + 1) * (l + 2) // 2 for l in la), sum((l + 1) * (l + 2) // 2 for l in lb)
+        rows = off[a][:, None] + np.arange(na)[None, :]
+        cols = off[b][:, None] + np.arange(nb)[None, :]
+        blk = D[rows[:, :, None], cols[:, None, :]]
+        Rt = np.transpose(R, (0, 2, 1))
+        loc = np.transpose(atom_rot_batch(la, Rt), (0, 2, 1)) @ blk @ atom_rot_batch(lb, Rt)
+        out[key] = (F, loc.reshape(len(a), -1))
+    return out
+
+
 def collect(mols, densities):
     """Training samples per pair type from converged alpha densities."""
     data = {}
     for mol, D in zip(mols, densities):
-        idx, ls = _atom_functions(mol)
-        for key, a, b, f, R in pair_samples(mol):
-            blk = local_block(D, mol, a, b, R, idx, ls)
+        for key, (F, Y) in local_blocks(np.asarray(D), mol).items():
             data.setdefault(key, ([], []))
-            data[key][0].append(f)
-            data[key][1].append(blk.reshape(-1))
-    return {k: (np.array(x, dtype=np.float32), np.array(y, dtype=np.float32)) for k, (x, y) in data.items()}
+            data[key][0].append(F)
+            data[key][1].append(Y)
+    return {k: (np.concatenate(x).astype(np.float32), np.concatenate(y).astype(np.float32))
+            for k, (x, y) in data.items()}
 
 
-def train(data, epochs=200, batch=1024, lr=2e-3, seed=0, time_budget=None, log=None):
-    """Train one network per pair type.  Returns {type: MLP}, {type: stats}."""
+def train(data, epochs=200, batch=1024, lr=2e-3, seed=0, time_budget=None, log=None, holdout=0.1):
+    """Train one network per pair type on standardised inputs and outputs
+    (Adam, learning rate halved every epochs/4), keeping a random fraction
+    ``holdout`` of each type's samples out for validation.  Returns
+    {type: MLP}, {type: stats}."""
     nets, stats = {}, {}
     t0 = time.perf_counter()
+    rng = np.random.default_rng(seed)
     for ti, (key, (X, Y)) in enumerate(sorted(data.items())):
+        order = rng.permutation(len(X))
+        nval = int(holdout * len(X)) if len(X) >= 20 else 0
+        val, tr = order[:nval], order[nval:]
         net = MLP(X.shape[1], Y.shape[1], seed=seed + ti)
-        net.xmean, net.xstd = X.mean(0), X.std(0) + 1e-6
-        net.ymean, net.ystd = Y.mean(0), Y.std(0) + 1e-6
-        Xd = torch.tensor((X - net.xmean) / net.xstd, device="cuda")
-        Yd = torch.tensor((Y - net.ymean) / net.ystd, device="cuda")
-        n = Xd.shape[0]
-        g = torch.Generator(device="cpu").manual_seed(seed)
+        net.xmean, net.xstd = X[tr].mean(0), X[tr].std(0) + 1e-6
+        net.ymean, net.ystd = Y[tr].mean(0), Y[tr].std(0) + 1e-6
+        Xd = torch.tensor((X - net.xmean) / net.xstd, dtype=torch.float32, device="cuda")
+        Yd = torch.tensor((Y - net.ymean) / net.ystd, dtype=torch.float32, device="cuda")
+        tr_d = torch.as_tensor(tr, device="cuda")
+        n = len(tr)
+        g = torch.Generator(device="cpu").manual_seed(seed + ti)
         first = last = None
         for ep in range(epochs):
-            perm = torch.randperm(n, generator=g).to("cuda")
-            tot = 0.0
+            perm = tr_d[torch.randperm(n, generator=g).to("cuda")]
+            tot = torch.zeros(1, dtype=torch.float32, device="cuda")
             nb = 0
-            for s in range(0, n, batch):
-                ids = perm[s:s + batch]
+            for s0 in range(0, n, batch):
+                ids = perm[s0:s0 + batch]
                 xb, yb = Xd[ids].contiguous(), Yd[ids].contiguous()
                 out, acts = net.forward(xb, keep=True)
                 loss, dY = net.loss_grad(out, yb)
                 gW, gb = net.backward(acts, dY)
                 net.adam(gW, gb, lr=lr * (0.5 ** (ep / max(1, epochs // 4))))
-                tot += float(loss)
+                tot += loss
                 nb += 1
-            last = tot / nb
+            last = float(tot) / nb
             first = last if first is None else first
             if time_budget and time.perf_counter() - t0 > time_budget * (ti + 1) / len(data):
                 break
+        vloss = None
+        if nval:
+            vd = torch.as_tensor(val, device="cuda")
+            vout = net.forward(Xd[vd].contiguous())
+            vloss = float(((vout - Yd[vd]) ** 2).mean())
         nets[key] = net
-        stats["%s-%s%s" % (key[0], key[1], "-diag" if key[2] else "")] = {"samples": int(n), "epochs": ep + 1,
-                                                                          "loss_first": first, "loss_last": last}
+        stats["%s-%s%s" % (key[0], key[1], "-diag" if key[2] else "")] = {
+            "samples": int(n), "holdout": int(nval), "epochs": ep + 1, "loss_first": first, "loss_last": last,
+            "holdout_loss": vloss}
         if log:
-            log("type %s: %d samples, %d epochs, loss %.4g -> %.4g" % (key, n, ep + 1, first, last))
+            log("type %s: %d samples, %d epochs, loss %.4g -> %.4g, holdout %s"
+                % (key, n, ep + 1, first, last, "%.4g" % vloss if vloss is not None else "-"))
     return nets, stats
 
 
-def predict_density(mol, nets):
-    """Alpha-density guess assembled from the predicted local-frame blocks,
-    renormalised to N_alpha electrons and McWeeny-purified twice."""
-    idx, ls = _atom_functions(mol)
+def assemble_density(mol, blocks, groups=None, device="cuda"):
+    """Density (torch float64 on ``device``) from local-frame pair blocks
+    {type: [P, n_a n_b]} (torch or numpy): D_ab = M_a(R)^T D'_ab M_b(R), the
+    transpose for D_ba, the symmetric part on diagonal blocks.  Types missing
+    from ``blocks`` stay zero."""
+    groups = pair_data(mol) if groups is None else groups
+    off, ls = _atom_layout(mol)
     n = mol.NOrbitals
-    D = np.zeros((n, n))
-    groups = {}
-    for key, a, b, f, R in pair_samples(mol):
-        groups.setdefault(key, []).append((a, b, f, R))
-    for key, items in groups.items():
-        net = nets.get(key)
-        if net is None:
+    D = torch.zeros((n, n), dtype=torch.float64, device=device)
+    for key, (a, b, F, R) in groups.items():
+        if key not in blocks:
             continue
-        X = np.array([f for _, _, f, _ in items], dtype=np.float32)
-        Xd = torch.tensor((X - net.xmean) / net.xstd, device="cuda")
-        Y = net.forward(Xd).cpu().numpy() * net.ystd + net.ymean
-        for (a, b, f, R), y in zip(items, Y):
-            Ma, Mb = atom_rotation(ls[a], R), atom_rotation(ls[b], R)
-            blk = Ma.T @ y.reshape(len(idx[a]), len(idx[b])).astype(float) @ Mb
-            D[np.ix_(idx[a], idx[b])] = blk
-            if a != b:
-                D[np.ix_(idx[b], idx[a])] = blk.T
-            else:
-                D[np.ix_(idx[a], idx[a])] = 0.5 * (blk + blk.T)
+        la, lb = ls[a[0]], ls[b[0]]
+        na, nb = sum((l + 1) * (l + 2) // 2 for l in la), sum((l + 1) * (l + 2) // 2 for l in lb)
+        Y = torch.as_tensor(blocks[key], dtype=torch.float64, device=device)
+        Rt = torch.as_tensor(R, dtype=torch.float64, device=device)
+        G = atom_rot_batch(la, Rt).transpose(1, 2) @ Y.reshape(-1, na, nb) @ atom_rot_batch(lb, Rt)
+        rows = torch.as_tensor(off[a][:, None] + np.arange(na)[None, :], device=device)
+        cols = torch.as_tensor(off[b][:, None] + np.arange(nb)[None, :], device=device)
+        if key[2]:
+            D[rows[:, :, None], rows[:, None, :]] = 0.5 * (G + G.transpose(1, 2))
+        else:
+            D[rows[:, :, None], cols[:, None, :]] = G
+            D[cols[:, :, None], rows[:, None, :]] = G.transpose(1, 2)
     return D
 
 
+def predict_density(mol, nets, device="cuda"):
+    """Alpha-density guess from the networks' local-frame blocks (pair types
+    without a network stay zero)."""
+    groups = pair_data(mol)
+    blocks = {}
+    for key, (a, b, F, R) in groups.items():
+        net = nets.get(key)
+        if net is None:
+            continue
+        X = torch.tensor(((F - net.xmean) / net.xstd).astype(np.float32), device=device)
+        blocks[key] = net.forward(X).double() * torch.as_tensor(net.ystd, dtype=torch.float64, device=device) \
+            + torch.as_tensor(net.ymean, dtype=torch.float64, device=device)
+    return assemble_density(mol, blocks, groups, device)
+
+
 def purify(D, S, nalpha, rounds=2):
-    """Scale to tr(DS) = N_alpha, then McWeeny D <- 3 DSD - 2 DSDSD."""
-    D = D * (nalpha / max(1e-12, np.trace(D @ S)))
+    """Scale to tr(DS) = N_alpha, then McWeeny D <- 3 DSD - 2 DSDSD (numpy or
+    torch)."""
+    tr = (D * S).sum()
+    D = D * (nalpha / max(1e-12, float(tr)))
     for _ in range(rounds):
         DS = D @ S
         D = 3.0 * DS @ D - 2.0 * DS @ DS @ D
         D = 0.5 * (D + D.T)
     return D
+
+
+def guess_density(mol, nets, rounds=2):
+    """Learned guess for an SCF: predicted alpha density, purified against
+    the overlap on the device (needs mol.Overlap)."""
+    D = predict_density(mol, nets)
+    S = mol.Overlap.to(D.device) if isinstance(mol.Overlap, torch.Tensor) else torch.as_tensor(mol.Overlap, device=D.device)
+    return purify(D, S, mol.NAlphaElectrons, rounds)
 
 
 def save(nets, path, meta=None):
@@ -383,6 +568,32 @@ def load(path):
         a, b, diag = tag.split("_")
         nets[(a, b, bool(int(diag)))] = MLP.from_state(d)
     return nets
+
+
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "mlp_guess_631gs.npz")
+_DEFAULT_NETS = None
+
+
+def default_nets():
+    """The shipped networks (6-31G*, Cartesian d), loaded once; None when the
+    weights file is absent."""
+    global _DEFAULT_NETS
+    if _DEFAULT_NETS is None and os.path.exists(DEFAULT_WEIGHTS):
+        _DEFAULT_NETS = load(DEFAULT_WEIGHTS)
+    return _DEFAULT_NETS
+
+
+def learned_guess(mol, nets=None):
+    """Purified learned alpha density (torch, device) for ``mol``, or None
+    when the networks do not cover it (other basis, spherical d, or a pair
+    type without a network): the caller then falls back to SAD."""
+    nets = default_nets() if nets is None else nets
+    if not nets or mol.Basis != "631GS" or 2 not in mol.CartesianL:
+        return None
+    types = set(pair_data(mol).keys())
+    if not types <= set(nets):
+        return None
+    return guess_density(mol, nets)
 
 
 def training_molecules(count, seed=5, sigma=0.05):

@@ -20,10 +20,13 @@ import torch
 from .engine import mp2_energy
 
 
-def do(molecule, result=None, method="MP2", max_slab_bytes=0):
+def do(molecule, result=None, method="MP2", max_slab_bytes=0, precision="fp64"):
     """Correlation energy for the converged SCF ``result`` of
-    ``hartree_fock.do`` (uses molecule.Engine and its stored ERIs).  Returns
-    a dict with the correlation energy, the total energy and the parts."""
+    ``hartree_fock.do`` (uses molecule.Engine and its stored ERIs).
+    ``precision`` of the transform products: "fp64" or "bf16x3" (split-BF16
+    matrix-core products with FP32 accumulation; the energy sums stay FP64).
+    Returns a dict with the correlation energy, the total energy and the
+    parts."""
     eng = molecule.Engine
     st = molecule.State if result is None else None
     Ca = result["Ca"] if result else st.Alpha.MOs.numpy()
@@ -35,7 +38,7 @@ def do(molecule, result=None, method="MP2", max_slab_bytes=0):
     restricted = na == nb and np.array_equal(Ca, Cb)
     parts = {}
     if restricted:
-        ov = eng.mp2_ovov(Ca[:, :na], Ca[:, na:], Ca[:, :na], Ca[:, na:], max_slab_bytes)
+        ov = eng.mp2_ovov(Ca[:, :na], Ca[:, na:], Ca[:, :na], Ca[:, na:], max_slab_bytes, precision)
         e_cs = mp2_energy(ov, ea[:na], ea[na:], ea[:na], ea[na:], 0)
         e_os = mp2_energy(ov, ea[:na], ea[na:], ea[:na], ea[na:], 2)
         e_ss_half = mp2_energy(ov, ea[:na], ea[na:], ea[:na], ea[na:], 1)
@@ -43,14 +46,14 @@ def do(molecule, result=None, method="MP2", max_slab_bytes=0):
         del ov
         e_plain = e_cs
     else:
-        ov = eng.mp2_ovov(Ca[:, :na], Ca[:, na:], Ca[:, :na], Ca[:, na:], max_slab_bytes)
+        ov = eng.mp2_ovov(Ca[:, :na], Ca[:, na:], Ca[:, :na], Ca[:, na:], max_slab_bytes, precision)
         parts["aa"] = mp2_energy(ov, ea[:na], ea[na:], ea[:na], ea[na:], 1)
         del ov
         if nb > 0:
-            ov = eng.mp2_ovov(Cb[:, :nb], Cb[:, nb:], Cb[:, :nb], Cb[:, nb:], max_slab_bytes)
+            ov = eng.mp2_ovov(Cb[:, :nb], Cb[:, nb:], Cb[:, :nb], Cb[:, nb:], max_slab_bytes, precision)
             parts["bb"] = mp2_energy(ov, eb[:nb], eb[nb:], eb[:nb], eb[nb:], 1)
             del ov
-            ov = eng.mp2_ovov(Ca[:, :na], Ca[:, na:], Cb[:, :nb], Cb[:, nb:], max_slab_bytes)
+            ov = eng.mp2_ovov(Ca[:, :na], Ca[:, na:], Cb[:, :nb], Cb[:, nb:], max_slab_bytes, precision)
             parts["ab"] = mp2_energy(ov, ea[:na], ea[na:], eb[:nb], eb[nb:], 2)
             del ov
         else:

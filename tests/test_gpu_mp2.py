@@ -63,3 +63,50 @@ def test_ump2_vs_reference_spin_orbital_form(gpu):
     eri = m.Engine.eri_dense().cpu().numpy()
     e_ref = oscf.mp2_spin_orbital(eri, r["Ca"], r["Cb"], r["ea"], r["eb"], 2, 1)
     assert abs(res["correlation"] - e_ref) < 1e-10
+
+
+@pytest.mark.parametrize("shape", [(64, 128, 32), (1, 1, 1), (37, 130, 19), (200, 70, 333)])
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+def test_gemm_bf16x3_exact_on_bf16_operands(gpu, shape, ta, tb):
+    """Small-integer operands are exact in bf16 (lo parts zero) and their
+    products and sums exact in FP32: the split-BF16 product must equal the
+    FP64 one bit for bit, which pins the operand and result layouts."""
+    import torch
+    from pychem_amd.engine import gemm
+    M, N, K = shape
+    rng = np.random.default_rng(M * 3 + K)
+    A = rng.integers(-8, 9, size=(K, M) if ta else (M, K)).astype(float)
+    B = rng.integers(-8, 9, size=(N, K) if tb else (K, N)).astype(float)
+    C = gemm(torch.tensor(A, device="cuda"), torch.tensor(B, device="cuda"), ta, tb, precision="bf16x3").cpu().numpy()
+    ref = (A.T if ta else A) @ (B.T if tb else B)
+    assert np.array_equal(C, ref)
+
+
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+def test_gemm_bf16x3_accuracy(gpu, ta, tb):
+    """General operands: hi*hi + hi*lo + lo*hi with FP32 accumulation keeps
+    each product to ~2^-16 relative; the sum over K = 333 of O(1) terms stays
+    within 2e-4 relative to the largest element (2^-16 * sqrt(K) * margin)."""
+    import torch
+    from pychem_amd.engine import gemm
+    M, N, K = 150, 170, 333
+    rng = np.random.default_rng(5)
+    A = rng.normal(size=(K, M) if ta else (M, K))
+    B = rng.normal(size=(N, K) if tb else (K, N))
+    C = gemm(torch.tensor(A, device="cuda"), torch.tensor(B, device="cuda"), ta, tb, precision="bf16x3").cpu().numpy()
+    ref = (A.T if ta else A) @ (B.T if tb else B)
+    err = np.abs(C - ref).max() / np.abs(ref).max()
+    assert err < 2e-4
+    assert err > 0.0          # not silently the FP64 path
+
+
+def test_mp2_bf16x3_transform_energy_error(gpu):
+    """Butane RHF-MP2 with the split-BF16 transform: the correlation energy
+    stays within 1e-5 Eh of the FP64 transform's."""
+    from pychem_amd import hartree_fock, mp2
+    m = structures.Molecule(geometry.alkane(4, 0.02, 4010), basis="6-31G*", cartesian_l=[2])
+    r = hartree_fock.do(m, conv=1e-10, maxiter=100, guess="SAD")
+    e64 = mp2.do(m, r)["correlation"]
+    e3 = mp2.do(m, r, precision="bf16x3")["correlation"]
+    assert abs(e3 - e64) < 1e-5
+    assert e3 != e64

@@ -89,3 +89,20 @@ def test_reference_diis_oscillates_like_oracle(gpu):
     o = oscf.run_rhf(m, conv=1e-10, maxiter=12)
     assert not r["converged"] and not o["converged"]
     assert np.abs(np.array(r["history"]) - np.array(o["history"])).max() < 1e-7
+
+
+@pytest.mark.parametrize("m,nocc", [(7, 2), (194, 41), (300, 150)])
+def test_pq_purify_matches_eigenvector_projector(gpu, m, nocc):
+    """Device TC2 purification (pq_purify) against the projector built from
+    eigh: a random symmetric matrix with a gap of 0.5 between the occupied
+    and virtual blocks of its spectrum."""
+    import torch
+    from pychem_amd import hartree_fock
+    rng = np.random.default_rng(m)
+    Q = np.linalg.qr(rng.normal(size=(m, m)))[0]
+    ev = np.concatenate([np.sort(rng.uniform(-20.0, -0.3, nocc)), np.sort(rng.uniform(0.2, 6.0, m - nocc))])
+    F = (Q * ev) @ Q.T
+    P = hartree_fock.purify(torch.tensor(F, device="cuda"), nocc).cpu().numpy()
+    ref = Q[:, :nocc] @ Q[:, :nocc].T
+    assert np.abs(P - ref).max() < 1e-9
+    assert abs(np.trace(P) - nocc) < 1e-9
