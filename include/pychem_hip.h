@@ -94,8 +94,10 @@ int pq_eri_recompute(pq_basis* b, void* stream);
 /* Live kernel timing with hipEvents on the caller's stream (on != 0). */
 int pq_set_timing(pq_basis* b, int on);
 /* ms of the last timed ERI pass per class, eri_ms[2*nclass] (slot 2c: FP64
- * list, 2c+1: FP32 list, -1 if not launched), and of the last pq_jk call. */
-int pq_get_timings(pq_basis* b, double* eri_ms, double* jk_ms);
+ * list, 2c+1: FP32 list, -1 if not launched; the classes run concurrently on
+ * side streams), of the whole pass (eri_pass_ms, on the caller's stream) and
+ * of the last pq_jk call.  Outputs may be NULL. */
+int pq_get_timings(pq_basis* b, double* eri_ms, double* jk_ms, double* eri_pass_ms);
 /* Recompute one class of the stored ERIs: cls in [0, pq_eri_nclass()),
  * w = 0 for the FP64 list, 1 for the FP32 list. */
 int pq_eri_run_class(pq_basis* b, int cls, int w, void* stream);

@@ -41,7 +41,7 @@ SIGNATURES = {
     "pq_eri_compute": (c_int, [c_void_p, c_void_p]),
     "pq_eri_recompute": (c_int, [c_void_p, c_void_p]),
     "pq_set_timing": (c_int, [c_void_p, c_int]),
-    "pq_get_timings": (c_int, [c_void_p, P_double, P_double]),
+    "pq_get_timings": (c_int, [c_void_p, P_double, P_double, P_double]),
     "pq_eri_run_class": (c_int, [c_void_p, c_int, c_int, c_void_p]),
     "pq_eri_nclass": (c_int, []),
     "pq_eri_stats": (c_int, [c_void_p, P_ll, c_int]),

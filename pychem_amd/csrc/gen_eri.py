@@ -383,6 +383,9 @@ def main():
         for rn, rt in (("d", "double"), ("f", "float")):
             src.append("void eri_run_%s_%s(const EriArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
             src.append("{ hipLaunchKernelGGL(eri_k%s<%s>, dim3((a.nq + ERI_BLOCK - 1) / ERI_BLOCK), dim3(ERI_BLOCK), 0, st, a); }" % (tag, rt))
+            src.append("size_t eri_scratch_%s_%s()" % (tag, rn))
+            src.append("{ hipFuncAttributes f; return hipFuncGetAttributes(&f, (const void*)eri_k%s<%s>) == hipSuccess ? "
+                       "f.localSizeBytes : (size_t)1 << 30; }" % (tag, rt))
             jsrc.append("void jk_run_%s_%s(const JkArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
             jsrc.append("{ hipLaunchKernelGGL((jk_kernel<%d, %d, %d, %d, %s>), dim3((a.nq + 127) / 128), dim3(128), 0, st, a); }" % (la, lb, lc, ld, rt))
             jsrc.append("void scatter_run_%s_%s(const ScatterArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
@@ -413,6 +416,7 @@ def main():
         tag = "%d%d%d%d" % (la, lb, lc, ld)
         for rn, rt in (("d", "double"), ("f", "float")):
             d.append("void eri_run_%s_%s(const EriArgs<%s>&, hipStream_t);" % (tag, rn, rt))
+            d.append("size_t eri_scratch_%s_%s();" % (tag, rn))
             d.append("void jk_run_%s_%s(const JkArgs<%s>&, hipStream_t);" % (tag, rn, rt))
             d.append("void scatter_run_%s_%s(const ScatterArgs<%s>&, hipStream_t);" % (tag, rn, rt))
     d.append("void onee_run(int pcls, const OneeArgs& a, hipStream_t st);")
@@ -421,6 +425,11 @@ def main():
     for la, lb, lc, ld, ncomp, X, Y, nbody in table:
         d.append("  {%d, %d, %d, %d, %d, %d, %d}," % (la, lb, lc, ld, ncomp, X, Y))
     d.append("};")
+    d.append("// per-lane scratch (spill) bytes of each class kernel, FP64 and FP32 forms")
+    for rn in ("d", "f"):
+        d.append("static size_t (*const eri_scratch_table_%s[ERI_NCLASS])() = {" % rn)
+        d.append("  " + ", ".join("eri_scratch_%d%d%d%d_%s" % (t[0], t[1], t[2], t[3], rn) for t in table))
+        d.append("};")
     d.append("// tiled J/K: bra tile TA x TB, tiles per quartet, LDS doubles per basis function (2 (TA + TB))")
     d.append("static const int jk_tile_table[ERI_NCLASS][4] = {")
     for la, lb, lc, ld, ncomp, X, Y, nbody in table:

@@ -168,7 +168,8 @@ struct DevBuf {
 
 struct EriClassStore {
     DevBuf q, vals, items, kets, jkbuf, jtiles;
-    int njtile = 0;            // ket J reduction tiles
+    std::vector<int4> hjtiles; // host copy of the ket J reduction tiles
+    int njtile = 0;
     // per bra of the class's bra list (see lists.h)
     DevBuf j0, cnt, np, qs, nit, its;
     int nitems = 0;
@@ -197,10 +198,12 @@ struct pq_basis {
     // per pair class: pair ids sorted by (molecule, bound desc), molecule
     // segment starts, cumulative primitive-pair counts (device)
     DevBuf d_sorted[6], d_seg[6], d_npcum[6], d_sbound[6], d_total;
-    std::vector<int> hseg[6];                // host copies: molecule segments, sorted bounds
+    std::vector<int> hseg[6], hsorted[6];    // host copies of what the device lists read
     std::vector<double> hsbound[6];
+    std::vector<long long> hnpcum[6];
     // every tiled list's ket J reduction, launched once per J/K call
     DevBuf d_jred;
+    std::vector<JredList> hjred;
     int njred = 0;
     long long jred_blocks = 0;
     int nsorted[6] = {};
@@ -214,6 +217,8 @@ struct pq_basis {
     int ev_cls_used[2 * ERI_NCLASS] = {};
     hipEvent_t ev_jk[2] = {};
     int ev_jk_used = 0;
+    hipEvent_t ev_pass[2] = {};   // whole ERI pass on the caller's stream
+    int ev_pass_used = 0;
     hipGraphExec_t graph_exec = nullptr;
     // J/K class launches fan out over side streams (default priority)
     int jk_nstreams = 4;
@@ -239,6 +244,8 @@ static int ensure_events(pq_basis* b)
         for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreate(&b->ev_cls[i][k]));
     HIPCHK(hipEventCreate(&b->ev_jk[0]));
     HIPCHK(hipEventCreate(&b->ev_jk[1]));
+    HIPCHK(hipEventCreate(&b->ev_pass[0]));
+    HIPCHK(hipEventCreate(&b->ev_pass[1]));
     return 0;
 }
 
@@ -439,6 +446,8 @@ extern "C" void pq_basis_destroy(pq_basis* b)
             for (int k = 0; k < 2; ++k) (void)hipEventDestroy(b->ev_cls[i][k]);
         (void)hipEventDestroy(b->ev_jk[0]);
         (void)hipEventDestroy(b->ev_jk[1]);
+        (void)hipEventDestroy(b->ev_pass[0]);
+        (void)hipEventDestroy(b->ev_pass[1]);
     }
     if (b->ev_fork) {
         for (int k = 0; k < 4; ++k) {
@@ -548,28 +557,49 @@ extern "C" int pq_schwarz(pq_basis* b, double* pair_bound_host, void* stream)
 {
     if (!b) return fail("pq_schwarz: null basis");
     hipStream_t st = (hipStream_t)stream;
+    // one upload and one scratch area for the six diagonal classes (P|P)
+    size_t nq_all = 0, nv_all = 0;
     for (int X = 0; X < 6; ++X) {
-        const std::vector<int>& pl = b->class_pairs[X];
-        const int n = (int)pl.size();
+        const size_t n = b->class_pairs[X].size();
+        nq_all += n;
+        nv_all += n * eri_class_table[class_of_pair_classes(X, X)][4];
+    }
+    std::vector<int2> hq;
+    std::vector<int> hids;
+    hq.reserve(nq_all);
+    hids.reserve(nq_all);
+    for (int X = 0; X < 6; ++X)
+        for (int id : b->class_pairs[X]) {
+            hq.push_back(make_int2(id, id));
+            hids.push_back(id);
+        }
+    DevBuf dq, dv, dids;
+    if (dq.ensure(sizeof(int2) * std::max<size_t>(1, nq_all)) || dv.ensure(sizeof(double) * std::max<size_t>(1, nv_all)) ||
+        dids.ensure(sizeof(int) * std::max<size_t>(1, nq_all)))
+        return -1;
+    if (nq_all) {
+        HIPCHK(hipMemcpyAsync(dq.p, hq.data(), sizeof(int2) * nq_all, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dids.p, hids.data(), sizeof(int) * nq_all, hipMemcpyHostToDevice, st));
+    }
+    size_t oq = 0, ov = 0;
+    for (int X = 0; X < 6; ++X) {
+        const int n = (int)b->class_pairs[X].size();
         if (!n) continue;
         const int cls = class_of_pair_classes(X, X);
-        std::vector<int2> hq(n);
-        for (int i = 0; i < n; ++i) hq[i] = make_int2(pl[i], pl[i]);
-        DevBuf dq, dv, dids;
         const int ncomp = eri_class_table[cls][4];
-        if (dq.ensure(sizeof(int2) * n) || dv.ensure(sizeof(double) * (size_t)n * ncomp) || dids.ensure(sizeof(int) * n))
-            return -1;
-        HIPCHK(hipMemcpyAsync(dq.p, hq.data(), sizeof(int2) * n, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(dids.p, pl.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-        if (run_class<double>(b, cls, (const int2*)dq.p, n, (double*)dv.p, st)) return -1;
+        const int2* q = (const int2*)dq.p + oq;
+        double* v = (double*)dv.p + ov;
+        if (run_class<double>(b, cls, q, n, v, st)) return -1;
         const int l1 = eri_class_table[cls][0], l2 = eri_class_table[cls][1];
-        hipLaunchKernelGGL(schwarz_extract_kernel, dim3((n + 127) / 128), dim3(128), 0, st, (const double*)dv.p,
-                           (const int*)dids.p, n, ncart_l(l1), ncart_l(l2), (const PairHdr*)b->d_pairs.p,
+        hipLaunchKernelGGL(schwarz_extract_kernel, dim3((n + 127) / 128), dim3(128), 0, st, (const double*)v,
+                           (const int*)dids.p + oq, n, ncart_l(l1), ncart_l(l2), (const PairHdr*)b->d_pairs.p,
                            b->any_spher ? (const int*)b->d_spher.p : (const int*)nullptr, (double*)b->d_bound.p);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(st));
-        dq.release(); dv.release(); dids.release();
+        oq += n;
+        ov += (size_t)n * ncomp;
     }
+    HIPCHK(hipStreamSynchronize(st));   // hq, hids and the scratch go out of scope
+    dq.release(); dv.release(); dids.release();
     b->bound.resize(b->npair);
     HIPCHK(hipMemcpy(b->bound.data(), b->d_bound.p, sizeof(double) * b->npair, hipMemcpyDeviceToHost));
     b->have_bound = 1;
@@ -604,15 +634,20 @@ static int upload_sorted_pairs(pq_basis* b, hipStream_t st)
         if (b->d_sorted[X].ensure(sizeof(int) * std::max(1, n)) || b->d_seg[X].ensure(sizeof(int) * seg.size()) ||
             b->d_npcum[X].ensure(sizeof(long long) * npcum.size()) || b->d_sbound[X].ensure(sizeof(double) * sb.size()))
             return -1;
-        HIPCHK(hipMemcpyAsync(b->d_sbound[X].p, sb.data(), sizeof(double) * sb.size(), hipMemcpyHostToDevice, st));
-        b->hseg[X] = seg;
-        b->hsbound[X] = sb;
-        if (n) HIPCHK(hipMemcpyAsync(b->d_sorted[X].p, sorted.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(b->d_seg[X].p, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(b->d_npcum[X].p, npcum.data(), sizeof(long long) * npcum.size(), hipMemcpyHostToDevice,
+        // the host copies live in the handle until the next plan, so the
+        // asynchronous uploads need no wait here
+        b->hsorted[X] = std::move(sorted);
+        b->hnpcum[X] = std::move(npcum);
+        b->hseg[X] = std::move(seg);
+        b->hsbound[X] = std::move(sb);
+        if (n)
+            HIPCHK(hipMemcpyAsync(b->d_sorted[X].p, b->hsorted[X].data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(b->d_seg[X].p, b->hseg[X].data(), sizeof(int) * b->hseg[X].size(), hipMemcpyHostToDevice,
                               st));
-        // the host vectors go out of scope: wait for the copies
-        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipMemcpyAsync(b->d_npcum[X].p, b->hnpcum[X].data(), sizeof(long long) * b->hnpcum[X].size(),
+                              hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(b->d_sbound[X].p, b->hsbound[X].data(), sizeof(double) * b->hsbound[X].size(),
+                              hipMemcpyHostToDevice, st));
     }
     return 0;
 }
@@ -651,16 +686,12 @@ static ListBras list_bras(EriClassStore& s)
 }
 
 // J/K work items of one list on the device: runs of equal bra pair cut into
-// pieces of at most jk_run quartets.  Automatic length: every item stages and
+// pieces of at most `run` quartets.  Automatic length: every item stages and
 // flushes 2 (TA + TB) rows of n elements, so its ERI values (run * ncomp)
 // should outweigh that four times over; capped so that the class keeps at
-// least 512 items, within 64..2048.  Synchronises `st` to learn the item
-// count.
-static int build_items(pq_basis* b, int c, int w, hipStream_t st)
+// least 512 items, within 64..2048.
+static long long item_run(const pq_basis* b, int c, const EriClassStore& s)
 {
-    EriClassStore& s = w ? b->store32[c] : b->store64[c];
-    s.nitems = 0;
-    if (!s.nq) return 0;
     long long run = s.jk_run;
     if (run <= 0) {
         const long long rows = jk_tile_table[c][3] / 2;
@@ -668,19 +699,41 @@ static int build_items(pq_basis* b, int c, int w, hipStream_t st)
         run = std::min(run, std::max(64LL, (long long)s.nq / 512));
         run = std::max(64LL, std::min(2048LL, (run + 63) / 64 * 64));
     }
-    const ListArgs a = list_args(b, c);
+    return run;
+}
+
+// Items per bra and their scan; the item count lands in *total (device).
+static void items_count(pq_basis* b, int c, int w, long long* total, hipStream_t st)
+{
+    EriClassStore& s = w ? b->store32[c] : b->store64[c];
     const ListBras lb = list_bras(s);
-    list_items_count(lb, a.nb, run, st);
-    list_scan(lb.nit, lb.its, a.nb, (long long*)b->d_total.p, st);
-    long long nitems = 0;
-    HIPCHK(hipMemcpyAsync(&nitems, b->d_total.p, sizeof(long long), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    const int nb = list_args(b, c).nb;
+    list_items_count(lb, nb, item_run(b, c, s), st);
+    list_scan(lb.nit, lb.its, nb, total, st);
+}
+
+static int items_fill(pq_basis* b, int c, int w, long long nitems, hipStream_t st)
+{
+    EriClassStore& s = w ? b->store32[c] : b->store64[c];
     if (nitems > 0x7fffffffLL) return fail("J/K work items exceed 2^31");
     s.nitems = (int)nitems;
     if (s.items.ensure(sizeof(JkItem) * std::max(1, s.nitems))) return -1;
-    list_items_fill(a, lb, nitems, run, (JkItem*)s.items.p, st);
+    list_items_fill(list_args(b, c), list_bras(s), nitems, item_run(b, c, s), (JkItem*)s.items.p, st);
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+// One list's items (synchronises `st` to learn the count).
+static int build_items(pq_basis* b, int c, int w, hipStream_t st)
+{
+    EriClassStore& s = w ? b->store32[c] : b->store64[c];
+    s.nitems = 0;
+    if (!s.nq) return 0;
+    items_count(b, c, w, (long long*)b->d_total.p, st);
+    long long nitems = 0;
+    HIPCHK(hipMemcpyAsync(&nitems, b->d_total.p, sizeof(long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return items_fill(b, c, w, nitems, st);
 }
 
 // The tiled kernel packs offsets into 15 bits and keeps 2 (TA + TB) rows of
@@ -753,9 +806,19 @@ extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold,
                 return -1;
             const ListBras lb = list_bras(s);
             list_fill(a, lb, s.nq, (int2*)s.q.p, (unsigned*)s.kets.p, st);
-            if (build_items(b, c, w, st)) return -1;
+            items_count(b, c, w, dtot + 2 * c + w, st);
         }
     }
+    // every list's item count in one transfer
+    std::vector<long long> nit(2 * ERI_NCLASS, 0);
+    HIPCHK(hipMemcpyAsync(nit.data(), dtot, sizeof(long long) * nit.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int c = 0; c < ERI_NCLASS; ++c)
+        for (int w = 0; w < 2; ++w) {
+            EriClassStore& s = w ? b->store32[c] : b->store64[c];
+            s.nitems = 0;
+            if (s.nq && items_fill(b, c, w, nit[2 * c + w], st)) return -1;
+        }
     // table of the ket J reductions of every tiled list
     std::vector<JredList> jr;
     long long blocks = 0;
@@ -766,13 +829,12 @@ extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold,
             EriClassStore& s = w ? b->store32[c] : b->store64[c];
             if (!s.nq) continue;
             const int X = eri_class_table[c][5], Y = eri_class_table[c][6];
-            const std::vector<int4> tl =
-                jred_tiles(b->hseg[X], b->hseg[Y], b->hsbound[X], b->hsbound[Y], X == Y, b->threshold);
+            s.hjtiles = jred_tiles(b->hseg[X], b->hseg[Y], b->hsbound[X], b->hsbound[Y], X == Y, b->threshold);
+            const std::vector<int4>& tl = s.hjtiles;   // kept in the handle: no wait for the upload
             s.njtile = (int)tl.size();
             if (s.jtiles.ensure(sizeof(int4) * std::max<size_t>(1, tl.size()))) return -1;
             if (!tl.empty())
                 HIPCHK(hipMemcpyAsync(s.jtiles.p, tl.data(), sizeof(int4) * tl.size(), hipMemcpyHostToDevice, st));
-            HIPCHK(hipStreamSynchronize(st));   // tl goes out of scope
             JredList L = jred_list(list_args(b, c), list_bras(s), w, (const double*)s.jkbuf.p, s.nq,
                                    jk_tile_table[c][2], n3 * n4, n4, (const int4*)s.jtiles.p, s.njtile);
             L.block0 = blocks;
@@ -784,34 +846,69 @@ extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold,
     b->njred = (int)jr.size();
     b->jred_blocks = blocks;
     if (b->d_jred.ensure(sizeof(JredList) * std::max<size_t>(1, jr.size()))) return -1;
-    if (!jr.empty())
-        HIPCHK(hipMemcpyAsync(b->d_jred.p, jr.data(), sizeof(JredList) * jr.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
+    b->hjred = std::move(jr);   // kept in the handle: no wait for the upload
+    if (!b->hjred.empty())
+        HIPCHK(hipMemcpyAsync(b->d_jred.p, b->hjred.data(), sizeof(JredList) * b->hjred.size(), hipMemcpyHostToDevice,
+                              st));
     HIPCHK(hipGetLastError());
     return 0;
 }
 
 // All class kernels over the planned lists (the ERI-generation pass).
+// The class kernels fork from `st` onto the side streams (costliest first,
+// each to the least-loaded stream; cost ~ primitive quartets x components) and
+// join back, so that the classes overlap.  Kernels that spill (per-lane
+// scratch: the runtime sizes scratch per hardware queue for a full device of
+// waves) stay on `st`, the one queue that always carried them.
 extern "C" int pq_eri_compute(pq_basis* b, void* stream)
 {
     if (!b) return fail("pq_eri_compute: null basis");
     hipStream_t st = (hipStream_t)stream;
     if (b->timing && ensure_events(b)) return -1;
-    for (int c = 0; c < ERI_NCLASS; ++c) {
+    std::vector<std::pair<double, int>> order;
+    for (int c = 0; c < ERI_NCLASS; ++c)
         for (int w = 0; w < 2; ++w) {
             EriClassStore& s = w ? b->store32[c] : b->store64[c];
-            const int slot = 2 * c + w;
-            b->ev_cls_used[slot] = 0;
-            if (!s.nq) continue;
-            if (b->timing) HIPCHK(hipEventRecord(b->ev_cls[slot][0], st));
-            int rc = w ? run_class<float>(b, c, (const int2*)s.q.p, s.nq, (float*)s.vals.p, st)
-                       : run_class<double>(b, c, (const int2*)s.q.p, s.nq, (double*)s.vals.p, st);
-            if (rc) return -1;
-            if (b->timing) {
-                HIPCHK(hipEventRecord(b->ev_cls[slot][1], st));
-                b->ev_cls_used[slot] = 1;
-            }
+            b->ev_cls_used[2 * c + w] = 0;
+            if (s.nq) order.push_back({(double)s.nprimq * s.ncomp + (double)s.nq * s.ncomp, 2 * c + w});
         }
+    std::sort(order.begin(), order.end(), [](const std::pair<double, int>& u, const std::pair<double, int>& v) {
+        return u.first > v.first;
+    });
+    const int ns = std::min(4, std::max(1, b->jk_nstreams));
+    if (b->timing) HIPCHK(hipEventRecord(b->ev_pass[0], st));
+    if (ns > 1) {
+        if (ensure_jk_streams(b)) return -1;
+        HIPCHK(hipEventRecord(b->ev_fork, st));
+        for (int k = 0; k < ns; ++k) HIPCHK(hipStreamWaitEvent(b->jk_side[k], b->ev_fork, 0));
+    }
+    double load[4] = {0.0, 0.0, 0.0, 0.0};
+    for (const auto& o : order) {
+        int k = 0;
+        for (int t = 1; t < ns; ++t)
+            if (load[t] < load[k]) k = t;
+        const int c = o.second / 2, w = o.second % 2, slot = o.second;
+        const bool spills = (w ? eri_scratch_table_f[c] : eri_scratch_table_d[c])() > 0;
+        if (!spills) load[k] += o.first;
+        hipStream_t sk = (ns > 1 && !spills) ? b->jk_side[k] : st;
+        EriClassStore& s = w ? b->store32[c] : b->store64[c];
+        if (b->timing) HIPCHK(hipEventRecord(b->ev_cls[slot][0], sk));
+        int rc = w ? run_class<float>(b, c, (const int2*)s.q.p, s.nq, (float*)s.vals.p, sk)
+                   : run_class<double>(b, c, (const int2*)s.q.p, s.nq, (double*)s.vals.p, sk);
+        if (rc) return -1;
+        if (b->timing) {
+            HIPCHK(hipEventRecord(b->ev_cls[slot][1], sk));
+            b->ev_cls_used[slot] = 1;
+        }
+    }
+    if (ns > 1)
+        for (int k = 0; k < ns; ++k) {
+            HIPCHK(hipEventRecord(b->ev_join[k], b->jk_side[k]));
+            HIPCHK(hipStreamWaitEvent(st, b->ev_join[k], 0));
+        }
+    if (b->timing) {
+        HIPCHK(hipEventRecord(b->ev_pass[1], st));
+        b->ev_pass_used = 1;
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -834,8 +931,9 @@ extern "C" int pq_set_timing(pq_basis* b, int on)
 }
 
 // Elapsed ms of the last timed ERI pass per class (slot 2c = FP64 list,
-// 2c+1 = FP32 list; -1 when not launched) and of the last pq_jk call.
-extern "C" int pq_get_timings(pq_basis* b, double* eri_ms, double* jk_ms)
+// 2c+1 = FP32 list; -1 when not launched; classes overlap on the side
+// streams), of the whole pass on the caller's stream, and of the last pq_jk.
+extern "C" int pq_get_timings(pq_basis* b, double* eri_ms, double* jk_ms, double* eri_pass_ms)
 {
     if (!b) return fail("pq_get_timings: null basis");
     if (!b->timing) return fail("pq_get_timings: timing not enabled");
@@ -848,6 +946,15 @@ extern "C" int pq_get_timings(pq_basis* b, double* eri_ms, double* jk_ms)
             HIPCHK(hipEventElapsedTime(&ms, b->ev_cls[i][0], b->ev_cls[i][1]));
             eri_ms[i] = ms;
         }
+    if (eri_pass_ms) {
+        *eri_pass_ms = -1.0;
+        if (b->ev_pass_used) {
+            HIPCHK(hipEventSynchronize(b->ev_pass[1]));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, b->ev_pass[0], b->ev_pass[1]));
+            *eri_pass_ms = ms;
+        }
+    }
     if (jk_ms) {
         *jk_ms = -1.0;
         if (b->ev_jk_used) {

@@ -99,12 +99,14 @@ class Engine:
         check(_lib.lib().pq_set_timing(self.h, 1 if on else 0), "pq_set_timing")
 
     def timings(self):
-        """(per-class ERI ms [nclass][2] of the last pass, ms of the last J/K)."""
+        """(per-class ERI ms [nclass][2] of the last pass (classes overlap on
+        the side streams), ms of the last J/K, ms of the whole last ERI pass)."""
         nc = _lib.lib().pq_eri_nclass()
         eri = np.zeros(2 * nc)
         jk = np.zeros(1)
-        check(_lib.lib().pq_get_timings(self.h, dptr(eri), dptr(jk)), "pq_get_timings")
-        return eri.reshape(nc, 2), float(jk[0])
+        tot = np.zeros(1)
+        check(_lib.lib().pq_get_timings(self.h, dptr(eri), dptr(jk), dptr(tot)), "pq_get_timings")
+        return eri.reshape(nc, 2), float(jk[0]), float(tot[0])
 
     def run_class(self, cls, fp32=False):
         check(_lib.lib().pq_eri_run_class(self.h, int(cls), 1 if fp32 else 0, _stream()), "pq_eri_run_class")
