@@ -140,16 +140,20 @@ struct JkItem {
 
 template <typename real>
 struct JkTileList {
-    const JkItem* items;
     const unsigned* kets;    // per quartet: o3 | o4 << 15 | (s3 == s4) << 30 | (P == Q) << 31
     const real* vals;
     double* jkbuf;           // [ntiles * N3 * N4][nq] ket J values per quartet
-    int nq, nitems;
+    int nq;
 };
 
 // Both stored lists of one class (FP64 values, and FP32 values below the
-// Schwarz split) in one launch: blocks take items of the FP64 list first.
+// Schwarz split) in one launch.  A work item is a chunk of one bra pair's
+// kets: its FP64 quartets [start, end) then its FP32 quartets items32[item]
+// (the bra's FP32 kets follow its FP64 kets in bound order), digested with
+// the same staged rows.
 struct JkTileArgs {
+    const JkItem* items;
+    const int2* items32;     // FP32 quartet range per item (NULL: FP64 list only)
     JkTileList<double> l64;
     JkTileList<float> l32;
     const MolInfo* mols;
@@ -157,50 +161,23 @@ struct JkTileArgs {
     const double* Dk;
     double* J;
     double* K;
-    int nmax, grid;          // grid: blocks in x (<= items; persistent when smaller)
+    int nmax, nitems, grid;  // grid: blocks in x (<= nitems; persistent when smaller)
 };
 
+// The quartets [q0, q1) of one list for the current item and tile.
 template <int LA, int LB, int LC, int LD, int TA, int TB, typename real>
-__device__ __forceinline__ void jk_tile_item(const JkTileList<real>& L, const JkTileArgs& a, int item, double* lds,
-                                             int tile)
+__device__ __forceinline__ void jk_tile_quartets(const JkTileList<real>& L, int q0, int q1, const double* drows,
+                                                 double* krows, int n, const double* __restrict__ Dj, double sbra,
+                                                 const double (&d12)[TA][TB], double (&j12)[TA][TB], int a0, int b0,
+                                                 int tile)
 {
-    constexpr int N1 = (LA + 1) * (LA + 2) / 2, N2 = (LB + 1) * (LB + 2) / 2;
+    constexpr int N2 = (LB + 1) * (LB + 2) / 2;
     constexpr int N3 = (LC + 1) * (LC + 2) / 2, N4 = (LD + 1) * (LD + 2) / 2;
-    static_assert(N1 % TA == 0 && N2 % TB == 0, "tile must divide the bra components");
-    constexpr int NTB = N2 / TB, NR = TA + TB;
-    const int a0 = (tile / NTB) * TA, b0 = (tile % NTB) * TB;
     const int tid = threadIdx.x, nt = blockDim.x;
     const size_t nq = L.nq;
     const real* __restrict__ vals = L.vals;
     double* __restrict__ jkbuf = L.jkbuf;
-    {
-        const JkItem it = L.items[item];
-        const MolInfo mi = a.mols[it.mol];
-        const int n = mi.n;
-        double* drows = lds;            // NR rows of Dk: o1+a0.., then o2+b0..
-        double* krows = lds + NR * n;   // K' rows, same order
-        const int o1 = it.bra & 0x7fff, o2 = (it.bra >> 15) & 0x7fff;
-        const double sbra = ((it.bra >> 30) & 1) ? 0.5 : 1.0;
-        const double* __restrict__ Dj = a.Dj + mi.base;
-        const double* __restrict__ Dk = a.Dk + mi.base;
-        double* const Jm = a.J + mi.base;
-        double* const Km = a.K + mi.base;
-        for (int i = tid; i < NR * n; i += nt) {
-            const int r = i / n, c = i - r * n;
-            const int row = r < TA ? o1 + a0 + r : o2 + b0 + (r - TA);
-            drows[i] = Dk[(size_t)row * n + c];
-            krows[i] = 0.0;
-        }
-        double d12[TA][TB], j12[TA][TB];
-#pragma unroll
-        for (int x = 0; x < TA; ++x)
-#pragma unroll
-            for (int y = 0; y < TB; ++y) {
-                d12[x][y] = Dj[(size_t)(o1 + a0 + x) * n + o2 + b0 + y];
-                j12[x][y] = 0.0;
-            }
-        __syncthreads();
-        for (int q = it.start + tid; q < it.end; q += nt) {
+    for (int q = q0 + tid; q < q1; q += nt) {
             const unsigned w = L.kets[q];
             const int o3 = w & 0x7fff, o4 = (w >> 15) & 0x7fff;
             double sc = sbra;
@@ -274,38 +251,75 @@ __device__ __forceinline__ void jk_tile_item(const JkTileList<real>& L, const Jk
 #pragma unroll
                 for (int y = 0; y < TB; ++y) atomicAdd(&krows[(TA + y) * n + o4 + d], 2.0 * k24[y][d]);
             }
-        }
-#pragma unroll
-        for (int x = 0; x < TA; ++x)
-#pragma unroll
-            for (int y = 0; y < TB; ++y) {
-                const double s = wave_sum(4.0 * j12[x][y]);
-                if ((tid & 63) == 0 && s != 0.0) atomic_addd(&Jm[(size_t)(o1 + a0 + x) * n + o2 + b0 + y], s);
             }
-        __syncthreads();
-        for (int i = tid; i < NR * n; i += nt) {
-            const double v = krows[i];
-            if (v != 0.0) {
-                const int r = i / n, c = i - r * n;
-                const int row = r < TA ? o1 + a0 + r : o2 + b0 + (r - TA);
-                atomic_addd(&Km[(size_t)row * n + c], v);
-            }
-        }
-        __syncthreads();
+}
+
+template <int LA, int LB, int LC, int LD, int TA, int TB>
+__device__ __forceinline__ void jk_tile_item(const JkTileArgs& a, int item, double* lds, int tile)
+{
+    constexpr int N1 = (LA + 1) * (LA + 2) / 2, N2 = (LB + 1) * (LB + 2) / 2;
+    static_assert(N1 % TA == 0 && N2 % TB == 0, "tile must divide the bra components");
+    constexpr int NTB = N2 / TB, NR = TA + TB;
+    const int a0 = (tile / NTB) * TA, b0 = (tile % NTB) * TB;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const JkItem it = a.items[item];
+    const MolInfo mi = a.mols[it.mol];
+    const int n = mi.n;
+    double* drows = lds;            // NR rows of Dk: o1+a0.., then o2+b0..
+    double* krows = lds + NR * n;   // K' rows, same order
+    const int o1 = it.bra & 0x7fff, o2 = (it.bra >> 15) & 0x7fff;
+    const double sbra = ((it.bra >> 30) & 1) ? 0.5 : 1.0;
+    const double* __restrict__ Dj = a.Dj + mi.base;
+    const double* __restrict__ Dk = a.Dk + mi.base;
+    double* const Jm = a.J + mi.base;
+    double* const Km = a.K + mi.base;
+    for (int i = tid; i < NR * n; i += nt) {
+        const int r = i / n, c = i - r * n;
+        const int row = r < TA ? o1 + a0 + r : o2 + b0 + (r - TA);
+        drows[i] = Dk[(size_t)row * n + c];
+        krows[i] = 0.0;
     }
+    double d12[TA][TB], j12[TA][TB];
+#pragma unroll
+    for (int x = 0; x < TA; ++x)
+#pragma unroll
+        for (int y = 0; y < TB; ++y) {
+            d12[x][y] = Dj[(size_t)(o1 + a0 + x) * n + o2 + b0 + y];
+            j12[x][y] = 0.0;
+        }
+    __syncthreads();
+    jk_tile_quartets<LA, LB, LC, LD, TA, TB, double>(a.l64, it.start, it.end, drows, krows, n, Dj, sbra, d12, j12,
+                                                     a0, b0, tile);
+    if (a.items32) {
+        const int2 r32 = a.items32[item];
+        jk_tile_quartets<LA, LB, LC, LD, TA, TB, float>(a.l32, r32.x, r32.y, drows, krows, n, Dj, sbra, d12, j12,
+                                                        a0, b0, tile);
+    }
+#pragma unroll
+    for (int x = 0; x < TA; ++x)
+#pragma unroll
+        for (int y = 0; y < TB; ++y) {
+            const double s = wave_sum(4.0 * j12[x][y]);
+            if ((tid & 63) == 0 && s != 0.0) atomic_addd(&Jm[(size_t)(o1 + a0 + x) * n + o2 + b0 + y], s);
+        }
+    __syncthreads();
+    for (int i = tid; i < NR * n; i += nt) {
+        const double v = krows[i];
+        if (v != 0.0) {
+            const int r = i / n, c = i - r * n;
+            const int row = r < TA ? o1 + a0 + r : o2 + b0 + (r - TA);
+            atomic_addd(&Km[(size_t)row * n + c], v);
+        }
+    }
+    __syncthreads();
 }
 
 template <int LA, int LB, int LC, int LD, int TA, int TB>
 __global__ void __launch_bounds__(256) jk_tile_kernel(const JkTileArgs a)
 {
     extern __shared__ double lds[];
-    const int total = a.l64.nitems + a.l32.nitems;
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        if (item < a.l64.nitems)
-            jk_tile_item<LA, LB, LC, LD, TA, TB, double>(a.l64, a, item, lds, blockIdx.y);
-        else
-            jk_tile_item<LA, LB, LC, LD, TA, TB, float>(a.l32, a, item - a.l64.nitems, lds, blockIdx.y);
-    }
+    for (int item = blockIdx.x; item < a.nitems; item += gridDim.x)
+        jk_tile_item<LA, LB, LC, LD, TA, TB>(a, item, lds, blockIdx.y);
 }
 
 // Dense scatter with 8-fold permutational symmetry (Methods/hartree_fock.py:314-325)

@@ -43,10 +43,13 @@ struct ListBras {
 void list_count(const ListArgs& a, const ListBras& b64, const ListBras& b32, hipStream_t st);
 // Exclusive scan of n values (in may equal out); *total = sum.
 void list_scan(const long long* in, long long* out, int n, long long* total, hipStream_t st);
-void list_items_count(const ListBras& b, int nb, long long run, hipStream_t st);
+// J/K work items of a class (nit / its of b64): per bra, its FP64 kets then
+// its FP32 kets (b32, may be NULL) in chunks of `run` quartets; items32 gets
+// each item's FP32 range.
+void list_items_count(const ListBras& b64, const ListBras* b32, int nb, long long run, hipStream_t st);
 void list_fill(const ListArgs& a, const ListBras& b, long long nq, int2* quartets, unsigned* kets, hipStream_t st);
-void list_items_fill(const ListArgs& a, const ListBras& b, long long nitems, long long run, JkItem* items,
-                     hipStream_t st);
+void list_items_fill(const ListArgs& a, const ListBras& b64, const ListBras* b32, long long nitems, long long run,
+                     JkItem* items, int2* items32, hipStream_t st);
 // J'_Q += sum over the bras that reach ket Q of the per-quartet ket J values
 // jkbuf[(tile * ncd + cd) * nq + q], over tiles of up to 64 kets x 256 bras of
 // one molecule (int4: first ket, kets, first bra, bras); one list per launch,

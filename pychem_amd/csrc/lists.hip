@@ -86,10 +86,13 @@ __global__ void __launch_bounds__(1024) list_scan_kernel(const long long* in, lo
     if (t == 1023) *total = part[1023];
 }
 
-__global__ void list_items_count_kernel(const long long* cnt, long long* nit, int nb, long long run)
+// Items of a class: per bra, its FP64 kets then its FP32 kets cut into
+// chunks of `run` quartets.
+__global__ void list_items_count_kernel(const long long* cnt64, const long long* cnt32, long long* nit, int nb,
+                                        long long run)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nb) nit[i] = (cnt[i] + run - 1) / run;
+    if (i < nb) nit[i] = (cnt64[i] + (cnt32 ? cnt32[i] : 0) + run - 1) / run;
 }
 
 // One thread per quartet: its bra is the last with qs <= q (bras without
@@ -107,22 +110,29 @@ __global__ void list_fill_kernel(const ListArgs a, const ListBras b, long long n
               ((kh.s1 == kh.s2) ? (1u << 30) : 0u) | ((P == Q) ? (1u << 31) : 0u);
 }
 
-__global__ void list_items_fill_kernel(const ListArgs a, const ListBras b, long long nitems, long long run,
-                                       JkItem* items)
+// Item t covers positions [lo, hi) of its bra's concatenated run (FP64
+// kets, then FP32 kets): FP64 quartets qs64 + [min(lo, c64), min(hi, c64)),
+// FP32 quartets qs32 + [max(lo - c64, 0), max(hi - c64, 0)).
+__global__ void list_items_fill_kernel(const ListArgs a, const ListBras b, const ListBras b32, long long nitems,
+                                       long long run, JkItem* items, int2* items32)
 {
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= nitems) return;
     const int i = last_le(b.its, a.nb, t);
     const long long r = t - b.its[i];
-    const long long start = b.qs[i] + r * run;
-    const long long stop = b.qs[i] + b.cnt[i];
+    const long long c64 = b.cnt[i], c32 = items32 ? b32.cnt[i] : 0;
+    const long long lo = r * run, hi = lo + run < c64 + c32 ? lo + run : c64 + c32;
     const PairHdr bh = a.pairs[a.bra[i]];
     JkItem it;
-    it.start = (int)start;
-    it.end = (int)(start + run < stop ? start + run : stop);
+    it.start = (int)(b.qs[i] + (lo < c64 ? lo : c64));
+    it.end = (int)(b.qs[i] + (hi < c64 ? hi : c64));
     it.bra = a.cart_off[bh.s1] | (a.cart_off[bh.s2] << 15) | ((bh.s1 == bh.s2) ? (1 << 30) : 0);
     it.mol = bh.mol;
     items[t] = it;
+    if (items32) {
+        const long long s0 = lo > c64 ? lo - c64 : 0, s1 = hi > c64 ? hi - c64 : 0;
+        items32[t] = make_int2((int)(b32.qs[i] + s0), (int)(b32.qs[i] + s1));
+    }
 }
 
 // Ket J reduction over tiles of up to 64 consecutive kets (one per lane) x up
@@ -234,11 +244,12 @@ void list_scan(const long long* in, long long* out, int n, long long* total, hip
     hipLaunchKernelGGL(list_scan_kernel, dim3(1), dim3(1024), 0, st, in, out, n, total);
 }
 
-void list_items_count(const ListBras& b, int nb, long long run, hipStream_t st)
+void list_items_count(const ListBras& b64, const ListBras* b32, int nb, long long run, hipStream_t st)
 {
     if (nb > 0)
-        hipLaunchKernelGGL(list_items_count_kernel, dim3(blocks_of(nb, 256)), dim3(256), 0, st, (const long long*)b.cnt,
-                           b.nit, nb, run);
+        hipLaunchKernelGGL(list_items_count_kernel, dim3(blocks_of(nb, 256)), dim3(256), 0, st,
+                           (const long long*)b64.cnt, b32 ? (const long long*)b32->cnt : (const long long*)nullptr,
+                           b64.nit, nb, run);
 }
 
 void list_fill(const ListArgs& a, const ListBras& b, long long nq, int2* quartets, unsigned* kets, hipStream_t st)
@@ -246,12 +257,12 @@ void list_fill(const ListArgs& a, const ListBras& b, long long nq, int2* quartet
     if (nq > 0) hipLaunchKernelGGL(list_fill_kernel, dim3(blocks_of(nq, 256)), dim3(256), 0, st, a, b, nq, quartets, kets);
 }
 
-void list_items_fill(const ListArgs& a, const ListBras& b, long long nitems, long long run, JkItem* items,
-                     hipStream_t st)
+void list_items_fill(const ListArgs& a, const ListBras& b64, const ListBras* b32, long long nitems, long long run,
+                     JkItem* items, int2* items32, hipStream_t st)
 {
     if (nitems > 0)
-        hipLaunchKernelGGL(list_items_fill_kernel, dim3(blocks_of(nitems, 256)), dim3(256), 0, st, a, b, nitems, run,
-                           items);
+        hipLaunchKernelGGL(list_items_fill_kernel, dim3(blocks_of(nitems, 256)), dim3(256), 0, st, a, b64,
+                           b32 ? *b32 : b64, nitems, run, items, b32 ? items32 : (int2*)nullptr);
 }
 
 // Tiles of one list: per molecule, runs of up to 64 kets x 256 bras; with

@@ -167,7 +167,7 @@ struct DevBuf {
 };
 
 struct EriClassStore {
-    DevBuf q, vals, items, kets, jkbuf, jtiles;
+    DevBuf q, vals, items, items32, kets, jkbuf, jtiles;   // items, items32: class level, in store64
     std::vector<int4> hjtiles; // host copy of the ket J reduction tiles
     int njtile = 0;
     // per bra of the class's bra list (see lists.h)
@@ -685,55 +685,61 @@ static ListBras list_bras(EriClassStore& s)
     return r;
 }
 
-// J/K work items of one list on the device: runs of equal bra pair cut into
-// pieces of at most `run` quartets.  Automatic length: every item stages and
-// flushes 2 (TA + TB) rows of n elements, so its ERI values (run * ncomp)
-// should outweigh that four times over; capped so that the class keeps at
-// least 512 items, within 64..2048.
-static long long item_run(const pq_basis* b, int c, const EriClassStore& s)
+// J/K work items of a class on the device (kept with the FP64 store): per bra
+// pair, its FP64 kets then its FP32 kets, cut into pieces of at most `run`
+// quartets.  Automatic length: every item stages and flushes 2 (TA + TB) rows
+// of n elements, so its ERI values (run * ncomp) should outweigh that four
+// times over; capped so that the class keeps at least 512 items, within
+// 64..2048.
+static long long item_run(const pq_basis* b, int c)
 {
+    const EriClassStore& s = b->store64[c];
+    const long long nq = (long long)b->store64[c].nq + b->store32[c].nq;
     long long run = s.jk_run;
     if (run <= 0) {
         const long long rows = jk_tile_table[c][3] / 2;
         run = (4LL * rows * b->nmax + s.ncomp - 1) / s.ncomp;
-        run = std::min(run, std::max(64LL, (long long)s.nq / 512));
+        run = std::min(run, std::max(64LL, nq / 512));
         run = std::max(64LL, std::min(2048LL, (run + 63) / 64 * 64));
     }
     return run;
 }
 
-// Items per bra and their scan; the item count lands in *total (device).
-static void items_count(pq_basis* b, int c, int w, long long* total, hipStream_t st)
+// Items per bra and their scan; the class's item count lands in *total.
+static void items_count(pq_basis* b, int c, long long* total, hipStream_t st)
 {
-    EriClassStore& s = w ? b->store32[c] : b->store64[c];
-    const ListBras lb = list_bras(s);
+    const ListBras l64 = list_bras(b->store64[c]), l32 = list_bras(b->store32[c]);
     const int nb = list_args(b, c).nb;
-    list_items_count(lb, nb, item_run(b, c, s), st);
-    list_scan(lb.nit, lb.its, nb, total, st);
+    list_items_count(l64, b->store32[c].nq ? &l32 : nullptr, nb, item_run(b, c), st);
+    list_scan(l64.nit, l64.its, nb, total, st);
 }
 
-static int items_fill(pq_basis* b, int c, int w, long long nitems, hipStream_t st)
+static int items_fill(pq_basis* b, int c, long long nitems, hipStream_t st)
 {
-    EriClassStore& s = w ? b->store32[c] : b->store64[c];
+    EriClassStore& s = b->store64[c];
     if (nitems > 0x7fffffffLL) return fail("J/K work items exceed 2^31");
     s.nitems = (int)nitems;
-    if (s.items.ensure(sizeof(JkItem) * std::max(1, s.nitems))) return -1;
-    list_items_fill(list_args(b, c), list_bras(s), nitems, item_run(b, c, s), (JkItem*)s.items.p, st);
+    const bool f32 = b->store32[c].nq > 0;
+    if (s.items.ensure(sizeof(JkItem) * std::max(1, s.nitems)) ||
+        (f32 && s.items32.ensure(sizeof(int2) * std::max(1, s.nitems))))
+        return -1;
+    const ListBras l32 = list_bras(b->store32[c]);
+    list_items_fill(list_args(b, c), list_bras(s), f32 ? &l32 : nullptr, nitems, item_run(b, c), (JkItem*)s.items.p,
+                    (int2*)s.items32.p, st);
     HIPCHK(hipGetLastError());
     return 0;
 }
 
-// One list's items (synchronises `st` to learn the count).
-static int build_items(pq_basis* b, int c, int w, hipStream_t st)
+// One class's items (synchronises `st` to learn the count).
+static int build_items(pq_basis* b, int c, hipStream_t st)
 {
-    EriClassStore& s = w ? b->store32[c] : b->store64[c];
-    s.nitems = 0;
-    if (!s.nq) return 0;
-    items_count(b, c, w, (long long*)b->d_total.p, st);
+    b->store64[c].nitems = 0;
+    if (!b->store64[c].nq && !b->store32[c].nq) return 0;
+    items_count(b, c, (long long*)b->d_total.p, st);
     long long nitems = 0;
     HIPCHK(hipMemcpyAsync(&nitems, b->d_total.p, sizeof(long long), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    return items_fill(b, c, w, nitems, st);
+    return items_fill(b, c, nitems, st);
 }
 
 // The tiled kernel packs offsets into 15 bits and keeps 2 (TA + TB) rows of
@@ -806,19 +812,17 @@ extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold,
                 return -1;
             const ListBras lb = list_bras(s);
             list_fill(a, lb, s.nq, (int2*)s.q.p, (unsigned*)s.kets.p, st);
-            items_count(b, c, w, dtot + 2 * c + w, st);
         }
+        if (b->store64[c].nq || b->store32[c].nq) items_count(b, c, dtot + c, st);
     }
-    // every list's item count in one transfer
-    std::vector<long long> nit(2 * ERI_NCLASS, 0);
+    // every class's item count in one transfer
+    std::vector<long long> nit(ERI_NCLASS, 0);
     HIPCHK(hipMemcpyAsync(nit.data(), dtot, sizeof(long long) * nit.size(), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    for (int c = 0; c < ERI_NCLASS; ++c)
-        for (int w = 0; w < 2; ++w) {
-            EriClassStore& s = w ? b->store32[c] : b->store64[c];
-            s.nitems = 0;
-            if (s.nq && items_fill(b, c, w, nit[2 * c + w], st)) return -1;
-        }
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        b->store64[c].nitems = b->store32[c].nitems = 0;
+        if ((b->store64[c].nq || b->store32[c].nq) && items_fill(b, c, nit[c], st)) return -1;
+    }
     // table of the ket J reductions of every tiled list
     std::vector<JredList> jr;
     long long blocks = 0;
@@ -1015,17 +1019,20 @@ static int jk_launch_class(pq_basis* b, int c, const double* Dj, const double* D
     if (jk_tile_fits(b, c)) {
         if (part & 1) {
             JkTileArgs a;
-            a.l64 = JkTileList<double>{(const JkItem*)s64.items.p, (const unsigned*)s64.kets.p,
-                                       (const double*)s64.vals.p, (double*)s64.jkbuf.p, s64.nq, u64 ? s64.nitems : 0};
-            a.l32 = JkTileList<float>{(const JkItem*)s32.items.p, (const unsigned*)s32.kets.p,
-                                      (const float*)s32.vals.p, (double*)s32.jkbuf.p, s32.nq, u32 ? s32.nitems : 0};
+            a.items = (const JkItem*)s64.items.p;
+            a.items32 = u32 ? (const int2*)s64.items32.p : (const int2*)nullptr;
+            a.l64 = JkTileList<double>{(const unsigned*)s64.kets.p, (const double*)s64.vals.p, (double*)s64.jkbuf.p,
+                                       s64.nq};
+            a.l32 = JkTileList<float>{(const unsigned*)s32.kets.p, (const float*)s32.vals.p, (double*)s32.jkbuf.p,
+                                      s32.nq};
             a.mols = dm;
             a.Dj = Dj;
             a.Dk = Dk;
             a.J = J;
             a.K = K;
             a.nmax = b->nmax;
-            const int items = a.l64.nitems + a.l32.nitems;
+            const int items = s64.nitems;
+            a.nitems = items;
             const int tuned = s64.jk_grid > 0 ? s64.jk_grid : s32.jk_grid;
             a.grid = (tuned > 0 && tuned < items) ? tuned : items;
             const double mean = (double)((u64 ? s64.nq : 0) + (u32 ? s32.nq : 0)) / std::max(1, items);
@@ -1127,9 +1134,8 @@ extern "C" int pq_jk_set_params(pq_basis* b, int cls, int run_len, int grid, voi
         EriClassStore& s = w ? b->store32[cls] : b->store64[cls];
         s.jk_run = run_len;
         s.jk_grid = grid;
-        if (!s.nq) continue;
-        if (build_items(b, cls, w, st)) return -1;
     }
+    if (build_items(b, cls, st)) return -1;
     HIPCHK(hipStreamSynchronize(st));
     return 0;
 }
