@@ -164,10 +164,14 @@ struct JkTileArgs {
     int nmax, nitems, grid;  // grid: blocks in x (<= nitems; persistent when smaller)
 };
 
-// The quartets [q0, q1) of one list for the current item and tile.
+// The quartets [q0, q1) of one list for the current item and tile.  Quartet
+// q0 + i goes to thread (i + shift) mod blockDim: with shift = the number of
+// quartets of the FP64 range, the FP32 range continues where the FP64 range
+// ended, so that in a short item no thread takes a quartet of each list.
 template <int LA, int LB, int LC, int LD, int TA, int TB, typename real>
-__device__ __forceinline__ void jk_tile_quartets(const JkTileList<real>& L, int q0, int q1, const double* drows,
-                                                 double* krows, int n, const double* __restrict__ Dj, double sbra,
+__device__ __forceinline__ void jk_tile_quartets(const JkTileList<real>& L, int q0, int q1, int shift,
+                                                 const double* drows, double* krows, int n,
+                                                 const double* __restrict__ Dj, double sbra,
                                                  const double (&d12)[TA][TB], double (&j12)[TA][TB], int a0, int b0,
                                                  int tile)
 {
@@ -177,7 +181,8 @@ __device__ __forceinline__ void jk_tile_quartets(const JkTileList<real>& L, int 
     const size_t nq = L.nq;
     const real* __restrict__ vals = L.vals;
     double* __restrict__ jkbuf = L.jkbuf;
-    for (int q = q0 + tid; q < q1; q += nt) {
+    const int first = (tid - shift % nt + nt) % nt;
+    for (int q = q0 + first; q < q1; q += nt) {
             const unsigned w = L.kets[q];
             const int o3 = w & 0x7fff, o4 = (w >> 15) & 0x7fff;
             double sc = sbra;
@@ -201,7 +206,11 @@ __device__ __forceinline__ void jk_tile_quartets(const JkTileList<real>& L, int 
             for (int c = 0; c < N3; ++c) {
                 // all loads of this ket row first (values and Dj elements), so
                 // that they are in flight together
-                double vv[N4][TA][TB], dj[N4];
+                // values kept in the list's own type until use: an FP32 row
+                // batch takes half the registers, so its loads still fit in
+                // flight together
+                real vv[N4][TA][TB];
+                double dj[N4];
 #pragma unroll
                 for (int d = 0; d < N4; ++d) {
                     dj[d] = Dj[(size_t)(o3 + c) * n + o4 + d];
@@ -209,7 +218,7 @@ __device__ __forceinline__ void jk_tile_quartets(const JkTileList<real>& L, int 
                     for (int x = 0; x < TA; ++x)
 #pragma unroll
                         for (int y = 0; y < TB; ++y)
-                            vv[d][x][y] = (double)vals[(size_t)((((a0 + x) * N2 + b0 + y) * N3 + c) * N4 + d) * nq + q];
+                            vv[d][x][y] = vals[(size_t)((((a0 + x) * N2 + b0 + y) * N3 + c) * N4 + d) * nq + q];
                 }
                 double d13[TA], d23[TB], k13[TA], k23[TB];
 #pragma unroll
@@ -229,7 +238,7 @@ __device__ __forceinline__ void jk_tile_quartets(const JkTileList<real>& L, int 
                     for (int x = 0; x < TA; ++x)
 #pragma unroll
                         for (int y = 0; y < TB; ++y) {
-                            const double v = sc * vv[d][x][y];
+                            const double v = sc * (double)vv[d][x][y];
                             j12[x][y] += dj[d] * v;
                             jcd += d12[x][y] * v;
                             k13[x] += d24[y][d] * v;
@@ -288,12 +297,12 @@ __device__ __forceinline__ void jk_tile_item(const JkTileArgs& a, int item, doub
             j12[x][y] = 0.0;
         }
     __syncthreads();
-    jk_tile_quartets<LA, LB, LC, LD, TA, TB, double>(a.l64, it.start, it.end, drows, krows, n, Dj, sbra, d12, j12,
-                                                     a0, b0, tile);
+    jk_tile_quartets<LA, LB, LC, LD, TA, TB, double>(a.l64, it.start, it.end, 0, drows, krows, n, Dj, sbra, d12,
+                                                     j12, a0, b0, tile);
     if (a.items32) {
         const int2 r32 = a.items32[item];
-        jk_tile_quartets<LA, LB, LC, LD, TA, TB, float>(a.l32, r32.x, r32.y, drows, krows, n, Dj, sbra, d12, j12,
-                                                        a0, b0, tile);
+        jk_tile_quartets<LA, LB, LC, LD, TA, TB, float>(a.l32, r32.x, r32.y, it.end - it.start, drows, krows, n, Dj,
+                                                        sbra, d12, j12, a0, b0, tile);
     }
 #pragma unroll
     for (int x = 0; x < TA; ++x)

@@ -1004,7 +1004,6 @@ extern "C" int pq_eri_stats(const pq_basis* b, long long* stats, int nstats)
 
 extern "C" int pq_eri_nclass(void) { return ERI_NCLASS; }
 
-// One stored class (w = 0: FP64 list, 1: FP32 list) into J', K'.
 // One class into J', K'.  part: 1 = tiled digestion kernel, 2 = ket J
 // reductions, 3 = both.  lists: bit 0 the FP64 list, bit 1 the FP32 list; the
 // tiled kernel digests both in one launch.
