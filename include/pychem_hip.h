@@ -28,6 +28,13 @@ const char* pq_last_error(void);
 int pq_version(void);
 /* Number of visible GPUs. */
 int pq_device_count(void);
+/* Device memory cache of the library (no reference counterpart: the
+ * reference holds its integrals in host memory).  Buffers a basis releases
+ * stay cached for the next basis; pq_cache_release frees every cached block
+ * of the current device (synchronising it), pq_cache_bytes reports how many
+ * bytes are cached there. */
+int pq_cache_release(void);
+long long pq_cache_bytes(void);
 
 /* Boys function F_m(T), m = 0..mmax (mmax <= 8), out[n][mmax+1].
  * Replaces the interpolation in Methods/c_ints/two_electron_fundamentals.c:60-85
@@ -178,11 +185,14 @@ int pq_mp2_ovov(pq_basis* b, int o1, int v1, const double* Co1, const double* Cv
 /* Density projector of the device SCF loop: P = projector onto the nocc
  * lowest eigenvectors of the symmetric m x m matrix F (orthonormal basis),
  * by trace-correcting purification (X <- X^2 or 2X - X^2 from the Gershgorin-
- * scaled F; the products by rocBLAS dgemm).  Replaces the per-cycle
- * diagonalisation and occupation of Methods/hartree_fock.py:168-184 (same
- * density; orbitals are diagonalised once at convergence).  work: 2 m^2 + 4
- * doubles.  0 < nocc < m.  Stops once tr(X - X^2) < tol (one more step) or
- * after max_steps; steps_out / idem_out (host, may be NULL) report both. */
+ * scaled, symmetrised F; one fused product/trace/branch launch per step).
+ * Replaces the per-cycle diagonalisation and occupation of
+ * Methods/hartree_fock.py:168-184 (same density; orbitals are diagonalised
+ * once at convergence).  work: 4 ld^2 + 2 max_steps + 3 doubles, ld = m
+ * rounded up to a multiple of 32, 32-byte aligned.  0 < nocc < m,
+ * max_steps >= 1.  Stops once tr(X - X^2) < tol (one more step) or after
+ * max_steps; steps_out / idem_out (host, may be NULL) report the steps taken
+ * and tr(X - X^2) of the converged input. */
 int pq_purify(const double* F_dev, int m, int nocc, double* P_dev, double* work_dev, double tol, int max_steps,
               int* steps_out, double* idem_out, void* stream);
 

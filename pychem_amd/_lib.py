@@ -20,6 +20,8 @@ SIGNATURES = {
     "pq_last_error": (ctypes.c_char_p, []),
     "pq_version": (c_int, []),
     "pq_device_count": (c_int, []),
+    "pq_cache_release": (c_int, []),
+    "pq_cache_bytes": (ctypes.c_longlong, []),
     "pq_boys": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "pq_basis_create": (c_int, [c_int, P_int, P_int, P_int, P_double, P_double, P_double,
                                 c_int, P_double, P_double, ctypes.POINTER(c_void_p)]),

@@ -166,7 +166,7 @@ def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_grap
     X = X * valid[:, :, None]
     dropped = (~keep).to(torch.float64)                                  # [B, nm] dropped / pad columns
     nocc = torch.as_tensor([m.NAlphaElectrons for m in mols], device=dev)
-    enuc = torch.as_tensor([structures_nuclear(m) for m in mols], device=dev)
+    enuc = torch.as_tensor([structures_nuclear(m) for m in mols], dtype=torch.float64, device=dev)
     if guess == "SAD":
         D = 0.5 * _sad_padded(eng, mols)
     else:
@@ -230,7 +230,11 @@ def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_grap
             Fx = F
         Dn = _density(X, Fx, dropped, nocc)
         En = ((Dn * (Hp + Fx)) * vmask).sum((1, 2)) + enuc
-        newly = (~done) & ((En - E).abs() < conv) & (it > 1)
+        # |dE| < conv counts only once the orbital gradient of the input
+        # density has settled too: an oscillating SCF can pass through a small
+        # energy change far from convergence
+        rmax = R.abs().amax((1, 2))
+        newly = (~done) & ((En - E).abs() < conv) & (rmax <= conv ** 0.5) & (it > 1)
         D = torch.where(done[:, None, None], D, Dn)
         E = torch.where(done, E, En)
         iters = torch.where(done, iters, torch.full_like(iters, it))

@@ -14,8 +14,10 @@
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -143,24 +145,176 @@ static int fail(const std::string& m)
 static inline int ncart_l(int l) { return (l + 1) * (l + 2) / 2; }
 static inline int pair_class(int l1, int l2) { return l1 * (l1 + 1) / 2 + l2; }   // l1 >= l2
 
+// Device memory cache.  hipFree synchronises the whole device, and a basis
+// owns some hundred buffers, so creating and destroying bases (one per SCF,
+// one per element for the atomic guesses) would otherwise stall on every
+// buffer.  Released blocks go to a per-device pending list; they are handed
+// out again only after one device synchronisation done here, the first time
+// an allocation finds no free block that fits, so that no block still read or
+// written by queued work on any stream is reused.  Sizes are rounded to 1/8
+// of their power of two (waste below 12.5 %); a request takes a cached block
+// of at most 5/4 its rounded size.  hipMalloc failing releases the cache and
+// retries once.  pq_cache_release() returns everything to the driver; blocks
+// still cached at exit are left to the driver.  PQ_DEVICE_CACHE=0 in the
+// environment turns the cache off (plain hipMalloc / hipFree per buffer).
+namespace {
+constexpr int kCacheDevices = 64;
+struct DevCache {
+    std::mutex mu;
+    std::multimap<size_t, void*> avail[kCacheDevices], pending[kCacheDevices];
+    size_t held[kCacheDevices] = {};   // bytes cached (avail + pending)
+};
+DevCache& dev_cache()
+{
+    static DevCache* c = new DevCache();   // never destroyed: no HIP calls at exit
+    return *c;
+}
+size_t cache_round(size_t b)
+{
+    if (b <= 512) return 512;
+    size_t p = 1;
+    while (p * 2 <= b) p *= 2;              // largest power of two <= b
+    const size_t g = p >= 4096 ? p / 8 : 512;
+    return (b + g - 1) / g * g;
+}
+void* cache_take(std::multimap<size_t, void*>& m, size_t rb, size_t* got)
+{
+    auto it = m.lower_bound(rb);
+    if (it == m.end() || it->first > rb + rb / 4) return nullptr;
+    void* p = it->second;
+    *got = it->first;
+    m.erase(it);
+    return p;
+}
+bool cache_enabled()
+{
+    static const bool on = [] {
+        const char* e = getenv("PQ_DEVICE_CACHE");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+void cache_drop_all(DevCache& c, int dev)
+{
+    for (auto* m : {&c.avail[dev], &c.pending[dev]}) {
+        for (auto& kv : *m) (void)hipFree(kv.second);
+        m->clear();
+    }
+    c.held[dev] = 0;
+}
+}  // namespace
+
+static int dev_alloc(size_t b, void** out, size_t* cap, int* dev_out)
+{
+    *out = nullptr;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kCacheDevices) return fail("device memory cache: device index out of range");
+    if (!cache_enabled()) {
+        void* p = nullptr;
+        const hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        *out = p;
+        *cap = b;
+        *dev_out = dev;
+        return 0;
+    }
+    const size_t rb = cache_round(b);
+    DevCache& c = dev_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    size_t got = 0;
+    void* p = cache_take(c.avail[dev], rb, &got);
+    if (!p && !c.pending[dev].empty()) {
+        auto it = c.pending[dev].lower_bound(rb);
+        if (it != c.pending[dev].end() && it->first <= rb + rb / 4) {
+            HIPCHK(hipDeviceSynchronize());
+            c.avail[dev].insert(c.pending[dev].begin(), c.pending[dev].end());
+            c.pending[dev].clear();
+            p = cache_take(c.avail[dev], rb, &got);
+        }
+    }
+    if (p) {
+        c.held[dev] -= got;
+        *out = p;
+        *cap = got;
+        *dev_out = dev;
+        return 0;
+    }
+    hipError_t e = hipMalloc(&p, rb);
+    if (e == hipErrorOutOfMemory && c.held[dev]) {
+        (void)hipGetLastError();
+        HIPCHK(hipDeviceSynchronize());
+        cache_drop_all(c, dev);
+        e = hipMalloc(&p, rb);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    *out = p;
+    *cap = rb;
+    *dev_out = dev;
+    return 0;
+}
+
+static void dev_release(void* p, size_t cap, int dev)
+{
+    if (!p) return;
+    if (!cache_enabled()) {
+        (void)hipFree(p);
+        return;
+    }
+    DevCache& c = dev_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.pending[dev].insert({cap, p});
+    c.held[dev] += cap;
+}
+
+// Free every cached block of the current device (synchronises the device).
+extern "C" int pq_cache_release(void)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kCacheDevices) return fail("pq_cache_release: device index out of range");
+    DevCache& c = dev_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    HIPCHK(hipDeviceSynchronize());
+    cache_drop_all(c, dev);
+    return 0;
+}
+
+// Bytes of device memory the cache holds on the current device.
+extern "C" long long pq_cache_bytes(void)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kCacheDevices) return -1;
+    DevCache& c = dev_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    return (long long)c.held[dev];
+}
+
+// A device buffer that grows on demand (contents are not kept when it grows).
 struct DevBuf {
     void* p = nullptr;
-    size_t bytes = 0;
+    size_t bytes = 0;   // capacity of the block
+    int dev = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
     int ensure(size_t b)
     {
         if (b <= bytes && p) return 0;
-        if (p) hipFree(p);
-        p = nullptr;
-        bytes = 0;
+        release();
         if (b == 0) return 0;
-        hipError_t e = hipMalloc(&p, b);
-        if (e != hipSuccess) return fail(std::string("hipMalloc: ") + hipGetErrorString(e));
-        bytes = b;
-        return 0;
+        return dev_alloc(b, &p, &bytes, &dev);
     }
     void release()
     {
-        if (p) hipFree(p);
+        dev_release(p, bytes, dev);
         p = nullptr;
         bytes = 0;
     }
@@ -1426,27 +1580,32 @@ extern "C" int pq_purify(const double* F, int m, int nocc, double* P, double* wo
 {
     if (!F || !P || !work) return fail("pq_purify: null argument");
     if (m <= 0 || nocc <= 0 || nocc >= m) return fail("pq_purify: need 0 < nocc < m");
+    if (max_steps < 1) return fail("pq_purify: max_steps must be positive");
+    if ((uintptr_t)work % 32) return fail("pq_purify: work must be 32-byte aligned");
     hipStream_t st = (hipStream_t)stream;
-    double* X = work;
-    double* X2 = work + (size_t)m * m;
-    double* t = X2 + (size_t)m * m;
-    double* bounds = t + 2;
-    if (tc2_start(F, m, X, bounds, st)) return fail("pq_purify: launch failed");
-    int done = 0;
-    double th[2] = {0.0, 0.0};
-    bool last = false;
-    while (done < max_steps) {
-        const int k = last ? 1 : std::min(8, max_steps - done);
-        if (tc2_steps(X, X2, m, nocc, k, t, st)) return fail("pq_purify: launch failed");
-        done += k;
-        if (last) break;
-        HIPCHK(hipMemcpyAsync(th, t, sizeof th, hipMemcpyDeviceToHost, st));
+    Tc2Work w = tc2_work(work, m, max_steps);
+    if (tc2_start(F, w, st)) return fail("pq_purify: launch failed");
+    // Queue the step count of this thread's previous call (plus a margin),
+    // then 16 at a time: the steps record where to stop and the ones queued
+    // past it do nothing, so one host round trip usually suffices.
+    static thread_local int usual = 44;
+    std::vector<double> h(2 * (size_t)max_steps + 1);
+    int queued = 0, stop = 0x7fffffff;
+    int chunk = std::min(max_steps, usual + 4);
+    for (;;) {
+        if (tc2_steps(w, nocc, tol, queued, chunk, st)) return fail("pq_purify: launch failed");
+        queued += chunk;
+        HIPCHK(hipMemcpyAsync(h.data(), w.t, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        if (th[0] - th[1] < tol) last = true;
+        memcpy(&stop, &h[2 * (size_t)max_steps], sizeof stop);
+        if (stop < queued || queued >= max_steps) break;
+        chunk = std::min(16, max_steps - queued);
     }
-    if (tc2_finish(X, P, m, st)) return fail("pq_purify: launch failed");
-    if (steps_out) *steps_out = done;
-    if (idem_out) *idem_out = th[0] - th[1];
+    if (tc2_finish(w, nocc, queued, P, st)) return fail("pq_purify: launch failed");
+    const int last = std::min(stop, queued - 1);
+    usual = last + 1;
+    if (steps_out) *steps_out = last + 1;
+    if (idem_out) *idem_out = last >= 1 ? h[2 * (size_t)(last - 1)] - h[2 * (size_t)(last - 1) + 1] : 1.0;
     return 0;
 }
 

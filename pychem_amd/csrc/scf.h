@@ -2,10 +2,36 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-// X = (e_max - F)/(e_max - e_min) with Gershgorin bounds (bounds[2] scratch).
-int tc2_start(const double* F, int m, double* X, double* bounds, hipStream_t st);
-// `steps` TC2 steps on X (X2 and t[2] scratch; t ends as tr X, tr X^2 of the
-// last step's input).
-int tc2_steps(double* X, double* X2, int m, int nocc, int steps, double* t, hipStream_t st);
-// P = (X + X^T)/2.
-int tc2_finish(const double* X, double* P, int m, hipStream_t st);
+// Work layout of one purification (4 ld^2 + 2 max_steps + 3 doubles, ld = m
+// rounded up to 32): two (X, X^2) pairs of ld x ld matrices (zero outside
+// the m x m block) that alternate between steps, the Gershgorin bounds, per
+// step k the traces (tr X_k, tr X_k^2), and the stop word (index of the last
+// step).
+inline int tc2_ld(int m) { return (m + 31) / 32 * 32; }
+struct Tc2Work {
+    double* pair[2];    // pair[p]: X at +0, X^2 at +ld*ld
+    double* bounds;     // [2]
+    double* t;          // [2 * max_steps]
+    int* stop;          // one double's room, right after the traces
+    double* x0;         // start matrix (the X slot of pair 1 until step 1 overwrites it)
+    int m, ld, max_steps;
+};
+inline long long tc2_work_doubles(int m, int max_steps)
+{
+    const long long ld = tc2_ld(m);
+    return 4 * ld * ld + 2LL * max_steps + 3;
+}
+Tc2Work tc2_work(double* work, int m, int max_steps);
+
+// Gershgorin bounds, zeroed trace slots and X0 = (e_max - F)/(e_max - e_min)
+// (F symmetrised).
+int tc2_start(const double* F, Tc2Work& w, hipStream_t st);
+// Steps k0 .. k0 + steps - 1 (one launch each): step k forms X_k from the
+// pair and traces of step k - 1 (X0 for k = 0) and writes X_k, X_k^2 and
+// their traces into slot k.  The step after the one whose input has
+// tr(X - X^2) < tol is the last; it records its index in the stop word and
+// later queued steps do nothing.
+int tc2_steps(const Tc2Work& w, int nocc, double tol, int k0, int steps, hipStream_t st);
+// P = X_n symmetrised, n - 1 the recorded last step (or queued - 1 when no
+// step recorded one), X_n from step n - 1's pair and traces.
+int tc2_finish(const Tc2Work& w, int nocc, int queued, double* P, hipStream_t st);

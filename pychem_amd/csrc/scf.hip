@@ -4,29 +4,62 @@
 // (2002)) of a symmetric m x m matrix F in an orthonormal basis: X0 =
 // (e_max - F)/(e_max - e_min) with Gershgorin bounds, then each step
 // X <- X^2 or 2X - X^2, whichever brings tr X closer to the occupation.  The
-// fixed point is the projector onto the nocc lowest eigenvectors of F.  One
-// step is three launches (the product X X, one-block traces, elementwise
-// branch), with the branch taken on the device so that no host round trip is
-// needed between steps.  X X is a plain dense GEMM of a few hundred rows:
-// rocBLAS dgemm, one handle per thread.
+// fixed point is the projector onto the nocc lowest eigenvectors of F.
+//
+// One launch per step.  Step k reads the previous step's pair (X_{k-1},
+// X_{k-1}^2) and traces, forms X_k = X_{k-1}^2 or 2 X_{k-1} - X_{k-1}^2 on the
+// fly while loading operands, and writes the new pair (X_k, X_k^2) and
+// traces; pairs alternate between two buffers.  The matrices are kept with
+// a leading dimension ld = m rounded up to 32 and zero outside the m x m
+// block (zero rows and columns stay zero under X^2 and 2X - X^2 and add
+// nothing to the traces), so every tile is whole and every load is an
+// aligned 32-byte run without bounds checks.  X stays exactly symmetric
+// (the start symmetrises F, each step computes only the upper 32 x 32 output
+// tiles and mirrors them), so the right operand's columns are read as rows
+// of X too.  A block's eight waves each compute the whole 32 x 32 tile over a
+// share of k, straight from global memory with three 16-wide k blocks of
+// loads in flight at once, and one LDS pass sums the partial tiles.
+// Products on v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16], result register r of lane l is D[4r + l/16][l%16].  Sizes here are a few hundred rows, so a step is
+// latency-bound: no barrier inside the k loop, one launch per step.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include "scf.h"
 
-// bounds[0] = min_i (F_ii - r_i), bounds[1] = max_i (F_ii + r_i),
-// r_i = sum_{j != i} |F_ij| (one block).
-__global__ void __launch_bounds__(256) gersh_kernel(const double* __restrict__ F, int m, double* bounds)
+typedef double tc2_d4 __attribute__((ext_vector_type(4)));
+
+#define TC2_T 32   // output tile edge
+
+// Gershgorin interval of row i: lohi[i] = F_ii - r_i, lohi[m + i] = F_ii + r_i,
+// r_i = sum_{j != i} |F_ij| (one wave per row).
+__global__ void __launch_bounds__(256) gersh_rows_kernel(const double* __restrict__ F, int m, double* lohi)
+{
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= m) return;
+    double r = 0.0;
+    for (int j = lane; j < m; j += 64)
+        if (j != i) r += fabs(F[(size_t)i * m + j]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+    if (lane == 0) {
+        const double d = F[(size_t)i * m + i];
+        lohi[i] = d - r;
+        lohi[m + i] = d + r;
+    }
+}
+
+// bounds[0] = min_i lohi[i], bounds[1] = max_i lohi[m + i] (one block); also
+// zeroes the nt trace slots and clears the stop word.
+__global__ void __launch_bounds__(256) gersh_reduce_kernel(const double* __restrict__ lohi, int m, double* bounds,
+                                                           double* t, int nt, int* stop)
 {
     __shared__ double lo[256], hi[256];
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) t[i] = 0.0;
+    if (threadIdx.x == 0) *stop = 0x7fffffff;
     double mn = 1e300, mx = -1e300;
     for (int i = threadIdx.x; i < m; i += blockDim.x) {
-        double r = 0.0;
-        for (int j = 0; j < m; ++j)
-            if (j != i) r += fabs(F[(size_t)i * m + j]);
-        const double d = F[(size_t)i * m + i];
-        mn = fmin(mn, d - r);
-        mx = fmax(mx, d + r);
+        mn = fmin(mn, lohi[i]);
+        mx = fmax(mx, lohi[m + i]);
     }
     lo[threadIdx.x] = mn;
     hi[threadIdx.x] = mx;
@@ -44,101 +77,270 @@ __global__ void __launch_bounds__(256) gersh_kernel(const double* __restrict__ F
     }
 }
 
-__global__ void tc2_init_kernel(const double* __restrict__ F, int m, const double* __restrict__ bounds, double* X)
+// X0 = (e_max I - (F + F^T)/2) / (e_max - e_min) in the ld x ld layout, zero
+// outside the m x m block
+__global__ void tc2_init_kernel(const double* __restrict__ F, int m, int ld, const double* __restrict__ bounds,
+                                double* X)
 {
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= (long long)m * m) return;
-    const int i = (int)(t / m), j = (int)(t % m);
-    const double emin = bounds[0], emax = bounds[1];
-    X[t] = ((i == j ? emax : 0.0) - F[t]) / (emax - emin);
+    if (t >= (long long)ld * ld) return;
+    const int i = (int)(t / ld), j = (int)(t % ld);
+    double x = 0.0;
+    if (i < m && j < m) {
+        const double emin = bounds[0], emax = bounds[1];
+        const double f = 0.5 * (F[(size_t)i * m + j] + F[(size_t)j * m + i]);
+        x = ((i == j ? emax : 0.0) - f) / (emax - emin);
+    }
+    X[t] = x;
 }
 
-// t[0] = tr X, t[1] = tr X2 (one block)
-__global__ void __launch_bounds__(256) trace2_kernel(const double* __restrict__ X, const double* __restrict__ X2, int m,
-                                                      double* t)
+struct Tc2Step {
+    const double* P;      // X_{k-1}
+    const double* Q;      // X_{k-1}^2 (unused when direct)
+    const double* tin;    // tr X_{k-1}, tr X_{k-1}^2 (unused when direct)
+    double* X;            // X_k
+    double* X2;           // X_k^2
+    double* tout;         // += tr X_k, tr X_k^2 (zero on entry)
+    int* stop;            // index of the last step (0x7fffffff until known)
+    double nocc, tol;
+    int ld, nt;           // leading dimension (a multiple of 32), tiles per side
+    int k;                // this step
+    int direct;           // step 0: X_k = P
+};
+
+__device__ __forceinline__ double tc2_wave_sum(double v)
 {
-    __shared__ double a[256], b[256];
-    double s1 = 0.0, s2 = 0.0;
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
-        s1 += X[(size_t)i * m + i];
-        s2 += X2[(size_t)i * m + i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+#define TC2_W 8   // waves per block (each sums a share of k)
+#define TC2_C 3   // k blocks of 16 whose loads a wave has in flight together
+
+// DIRECT: X_k = P (step 0); otherwise X_k = Q when up, else 2 P - Q.  All
+// of a round's loads (both matrices) are issued before `up` is used, so the
+// branch costs no memory round trip of its own.
+template <bool DIRECT>
+__device__ __forceinline__ void tc2_step_body(const Tc2Step& a, bool up, double (&red)[TC2_W][4][4][64])
+{
+    // upper tile (bi, bj), bi <= bj, from the linear block index
+    int t = blockIdx.x, bi = 0;
+    while (t >= a.nt - bi) {
+        t -= a.nt - bi;
+        ++bi;
     }
-    a[threadIdx.x] = s1;
-    b[threadIdx.x] = s2;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            a[threadIdx.x] += a[threadIdx.x + s];
-            b[threadIdx.x] += b[threadIdx.x + s];
+    const int bj = bi + t;
+    const int i0 = bi * TC2_T, j0 = bj * TC2_T;
+    const int ld = a.ld;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lq = lane >> 4;
+    const double cp = up ? 0.0 : 2.0, cq = up ? 1.0 : -1.0;   // X_k = cp P + cq Q
+    // the block's own tile of X_k (and its mirror) is written at the end;
+    // its elements are loaded now, with the operands
+    constexpr int NX = TC2_T * TC2_T / (64 * TC2_W);
+    double xp[NX], xq[NX];
+#pragma unroll
+    for (int n = 0; n < NX; ++n) {
+        const int e = tid + 64 * TC2_W * n;
+        const size_t idx = (size_t)(i0 + (e >> 5)) * ld + j0 + (e & 31);
+        xp[n] = a.P[idx];
+        xq[n] = DIRECT ? 0.0 : a.Q[idx];
+    }
+    // Wave w sums the k blocks kb = w, w + TC2_W, ... of 16 k each, TC2_C of
+    // them per round with all their loads issued first.  Inside a block, MFMA
+    // step s pairs lane group q with k = 16 kb + 4 q + s, so a lane's four k
+    // values of one row are consecutive (one aligned 32-byte load; the
+    // product is a sum over k in any order, the same order for both
+    // operands).  Rows of the left operand: i0 + 16 u + li; columns of the
+    // right operand (read as rows of the symmetric X_k): j0 + 16 v + li.
+    // k blocks past the end read the last block again and are masked to zero.
+    const int nkb = ld / 16;
+    tc2_d4 acc[2][2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) acc[u][v] = (tc2_d4){0.0, 0.0, 0.0, 0.0};
+    for (int kb0 = wave; kb0 < nkb; kb0 += TC2_W * TC2_C) {
+        tc2_d4 pa[TC2_C][2], pb[TC2_C][2], qa[TC2_C][2], qb[TC2_C][2];
+        double keep[TC2_C];
+#pragma unroll
+        for (int c = 0; c < TC2_C; ++c) {
+            const int kb = kb0 + c * TC2_W;
+            const int k = 16 * min(kb, nkb - 1) + 4 * lq;
+            keep[c] = (double)(kb < nkb);
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const size_t ia = (size_t)(i0 + 16 * u + li) * ld + k, ib = (size_t)(j0 + 16 * u + li) * ld + k;
+                pa[c][u] = *(const tc2_d4*)(a.P + ia);
+                pb[c][u] = *(const tc2_d4*)(a.P + ib);
+                if (!DIRECT) {
+                    qa[c][u] = *(const tc2_d4*)(a.Q + ia);
+                    qb[c][u] = *(const tc2_d4*)(a.Q + ib);
+                }
+            }
         }
-        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < TC2_C; ++c) {
+            tc2_d4 fa[2], fb[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                fa[u] = DIRECT ? keep[c] * pa[c][u] : keep[c] * (cp * pa[c][u] + cq * qa[c][u]);
+                fb[u] = DIRECT ? keep[c] * pb[c][u] : keep[c] * (cp * pb[c][u] + cq * qb[c][u]);
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int v = 0; v < 2; ++v)
+                        acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u][s], fb[v][s], acc[u][v], 0, 0, 0);
+        }
     }
-    if (threadIdx.x == 0) {
-        t[0] = a[0];
-        t[1] = b[0];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) red[wave][2 * u + v][r][lane] = acc[u][v][r];
+    __syncthreads();
+    // output slot e = (sub-tile g, register r, lane l), 1024 of them, summed
+    // over the waves; sub-tile g: u = g / 2, v = g % 2
+    double trx = 0.0, trx2 = 0.0;
+    for (int e = tid; e < 1024; e += 64 * TC2_W) {
+        const int g = e >> 8, r = (e >> 6) & 3, l = e & 63, u = g >> 1, v = g & 1;
+        double x2 = 0.0;
+#pragma unroll
+        for (int w = 0; w < TC2_W; ++w) x2 += red[w][g][r][l];
+        const int gi = i0 + 16 * u + 4 * r + (l >> 4), gj = j0 + 16 * v + (l & 15);
+        a.X2[(size_t)gi * ld + gj] = x2;
+        if (bi != bj) a.X2[(size_t)gj * ld + gi] = x2;
+        else if (gi == gj) trx2 += x2;
+    }
+#pragma unroll
+    for (int n = 0; n < NX; ++n) {
+        const int e = tid + 64 * TC2_W * n;
+        const int gi = i0 + (e >> 5), gj = j0 + (e & 31);
+        const double x = DIRECT ? xp[n] : cp * xp[n] + cq * xq[n];
+        a.X[(size_t)gi * ld + gj] = x;
+        if (bi != bj) a.X[(size_t)gj * ld + gi] = x;
+        else if (gi == gj) trx += x;
+    }
+    if (bi == bj) {
+        trx = tc2_wave_sum(trx);
+        trx2 = tc2_wave_sum(trx2);
+        if (lane == 0) {
+            atomicAdd(&a.tout[0], trx);
+            atomicAdd(&a.tout[1], trx2);
+        }
     }
 }
 
-// X <- X2 when |tr X2 - nocc| < |2 tr X - tr X2 - nocc|, else 2 X - X2
-__global__ void tc2_select_kernel(double* X, const double* __restrict__ X2, long long n, double nocc,
-                                  const double* __restrict__ t)
+// Step k.  Once tr(X_{k-1} - X_{k-1}^2) < tol, step k is the last one (the
+// stop word records k); steps after the recorded one return at once, so the
+// host may queue more steps than turn out to be needed.
+__global__ void __launch_bounds__(64 * TC2_W) tc2_step_kernel(const Tc2Step a)
 {
-    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double t1 = t[0], t2 = t[1];
-    const bool up = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc);
-    const double x2 = X2[i];
-    X[i] = up ? x2 : 2.0 * X[i] - x2;
+    __shared__ double red[TC2_W][4][4][64];   // [wave][sub-tile][register][lane]
+    if (a.direct) {
+        tc2_step_body<true>(a, true, red);
+        return;
+    }
+    const int stop = *(volatile const int*)a.stop;
+    const double t1 = a.tin[0], t2 = a.tin[1];
+    if (stop < a.k) return;
+    if (t1 - t2 < a.tol && blockIdx.x == 0 && threadIdx.x == 0) *a.stop = a.k;
+    tc2_step_body<false>(a, fabs(t2 - a.nocc) < fabs(2.0 * t1 - t2 - a.nocc), red);
 }
 
-// P = (X + X^T) / 2
-__global__ void symmetrize_kernel(const double* __restrict__ X, double* P, int m)
+// P = sym(X_n) (m x m): n - 1 = the recorded last step, or the last queued
+// step when none was recorded; X_n from step n - 1's pair (ld layout) and
+// traces.
+__global__ void tc2_finish_kernel(const double* __restrict__ pair0, const double* __restrict__ pair1,
+                                  const double* __restrict__ tslots, const int* __restrict__ stop, int queued,
+                                  double nocc, int m, int ld, double* P)
 {
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= (long long)m * m) return;
     const int i = (int)(t / m), j = (int)(t % m);
-    P[t] = 0.5 * (X[(size_t)i * m + j] + X[(size_t)j * m + i]);
+    const int last = min(*stop, queued - 1);
+    const double* Xp = (last & 1) ? pair1 : pair0;
+    const double* X2p = Xp + (size_t)ld * ld;
+    const double t1 = tslots[2 * last], t2 = tslots[2 * last + 1];
+    const bool up = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc);
+    const size_t ij = (size_t)i * ld + j, ji = (size_t)j * ld + i;
+    const double a = up ? X2p[ij] : 2.0 * Xp[ij] - X2p[ij];
+    const double b = up ? X2p[ji] : 2.0 * Xp[ji] - X2p[ji];
+    P[t] = 0.5 * (a + b);
 }
 
 static inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
 
-int tc2_start(const double* F, int m, double* X, double* bounds, hipStream_t st)
+static inline double* tc2_X(const Tc2Work& w, int k) { return w.pair[k & 1]; }
+static inline double* tc2_X2(const Tc2Work& w, int k) { return w.pair[k & 1] + (size_t)w.ld * w.ld; }
+
+Tc2Work tc2_work(double* work, int m, int max_steps)
 {
-    if (m <= 0) return -1;
-    hipLaunchKernelGGL(gersh_kernel, dim3(1), dim3(256), 0, st, F, m, bounds);
-    hipLaunchKernelGGL(tc2_init_kernel, dim3(nblk((long long)m * m)), dim3(256), 0, st, F, m, (const double*)bounds, X);
+    Tc2Work w;
+    w.m = m;
+    w.ld = tc2_ld(m);
+    w.max_steps = max_steps;
+    const size_t ll = (size_t)w.ld * w.ld;
+    w.pair[0] = work;
+    w.pair[1] = work + 2 * ll;
+    w.bounds = work + 4 * ll;
+    w.t = w.bounds + 2;
+    w.stop = (int*)(w.t + 2 * max_steps);
+    w.x0 = nullptr;
+    return w;
+}
+
+int tc2_start(const double* F, Tc2Work& w, hipStream_t st)
+{
+    const int m = w.m, ld = w.ld;
+    if (m <= 0 || w.max_steps < 1) return -1;
+    // X0 goes into the X slot of pair 1, read by step 0 as its direct input;
+    // the X^2 slot of pair 1 is free until step 1: row intervals go there
+    w.x0 = w.pair[1];
+    double* lohi = w.pair[1] + (size_t)ld * ld;
+    hipLaunchKernelGGL(gersh_rows_kernel, dim3((m + 3) / 4), dim3(256), 0, st, F, m, lohi);
+    hipLaunchKernelGGL(gersh_reduce_kernel, dim3(1), dim3(256), 0, st, (const double*)lohi, m, w.bounds, w.t,
+                       2 * w.max_steps, w.stop);
+    hipLaunchKernelGGL(tc2_init_kernel, dim3(nblk((long long)ld * ld)), dim3(256), 0, st, F, m, ld,
+                       (const double*)w.bounds, w.x0);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-static rocblas_handle blas_handle()
+int tc2_steps(const Tc2Work& w, int nocc, double tol, int k0, int steps, hipStream_t st)
 {
-    static thread_local rocblas_handle h = nullptr;
-    if (!h && rocblas_create_handle(&h) != rocblas_status_success) h = nullptr;
-    return h;
-}
-
-int tc2_steps(double* X, double* X2, int m, int nocc, int steps, double* t, hipStream_t st)
-{
-    if (m <= 0 || steps < 0) return -1;
-    const long long n = (long long)m * m;
-    rocblas_handle h = blas_handle();
-    if (!h || rocblas_set_stream(h, st) != rocblas_status_success) return -1;
-    const double one = 1.0, zero = 0.0;
-    for (int s = 0; s < steps; ++s) {
-        // rocBLAS is column-major: it sees X^T and computes X^T X^T = (X X)^T,
-        // which read row-major is X X
-        if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, m, m, &one, X, m, X, m, &zero, X2,
-                          m) != rocblas_status_success)
-            return -1;
-        hipLaunchKernelGGL(trace2_kernel, dim3(1), dim3(256), 0, st, (const double*)X, (const double*)X2, m, t);
-        hipLaunchKernelGGL(tc2_select_kernel, dim3(nblk(n)), dim3(256), 0, st, X, (const double*)X2, n, (double)nocc,
-                           (const double*)t);
+    if (k0 < 0 || steps < 0 || k0 + steps > w.max_steps) return -1;
+    const int nt = w.ld / TC2_T;
+    for (int k = k0; k < k0 + steps; ++k) {
+        Tc2Step a;
+        a.direct = k == 0;
+        a.k = k;
+        a.P = a.direct ? w.x0 : tc2_X(w, k - 1);
+        a.Q = a.direct ? nullptr : tc2_X2(w, k - 1);
+        a.tin = a.direct ? nullptr : w.t + 2 * (k - 1);
+        a.X = tc2_X(w, k);
+        a.X2 = tc2_X2(w, k);
+        a.tout = w.t + 2 * k;
+        a.stop = w.stop;
+        a.nocc = (double)nocc;
+        a.tol = tol;
+        a.ld = w.ld;
+        a.nt = nt;
+        hipLaunchKernelGGL(tc2_step_kernel, dim3(nt * (nt + 1) / 2), dim3(64 * TC2_W), 0, st, a);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int tc2_finish(const double* X, double* P, int m, hipStream_t st)
+int tc2_finish(const Tc2Work& w, int nocc, int queued, double* P, hipStream_t st)
 {
-    hipLaunchKernelGGL(symmetrize_kernel, dim3(nblk((long long)m * m)), dim3(256), 0, st, X, P, m);
+    const int m = w.m;
+    if (queued < 1 || queued > w.max_steps) return -1;
+    hipLaunchKernelGGL(tc2_finish_kernel, dim3(nblk((long long)m * m)), dim3(256), 0, st,
+                       (const double*)w.pair[0], (const double*)w.pair[1], (const double*)w.t, (const int*)w.stop,
+                       queued, (double)nocc, m, w.ld, P);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

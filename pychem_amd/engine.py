@@ -184,6 +184,16 @@ class Engine:
         return out
 
 
+def release_cache():
+    """Return the library's cached device memory to the driver (bases keep
+    released buffers cached for the next basis)."""
+    check(_lib.lib().pq_cache_release(), "pq_cache_release")
+
+
+def cached_bytes():
+    return int(_lib.lib().pq_cache_bytes())
+
+
 def mp2_energy(ovov, eo1, ev1, eo2, ev2, mode):
     """MP2 energy reduction (mode 0 closed shell, 1 same spin, 2 opposite spin)."""
     dev = ovov.device

@@ -96,14 +96,16 @@ class ElectronicState:
 
 
 def nuclear_repulsion(molecule):
-    """sum_{i>j} Z_i Z_j / r_ij (Methods/integrals.py:196-207)."""
-    e = 0.0
-    for i, a in enumerate(molecule.Atoms):
-        for j in range(i):
-            b = molecule.Atoms[j]
-            r = np.linalg.norm(np.array(a.Coordinates) - np.array(b.Coordinates))
-            e += a.NuclearCharge * b.NuclearCharge / r
-    return e
+    """sum_{i>j} Z_i Z_j / r_ij (Methods/integrals.py:196-207), over all atom
+    pairs at once."""
+    atoms = molecule.Atoms
+    if len(atoms) < 2:
+        return 0.0
+    R = np.array([a.Coordinates for a in atoms], dtype=np.float64)
+    Z = np.array([a.NuclearCharge for a in atoms], dtype=np.float64)
+    i, j = np.tril_indices(len(atoms), -1)
+    r = np.sqrt(((R[i] - R[j]) ** 2).sum(1))
+    return float((Z[i] * Z[j] / r).sum())
 
 
 def get_engine(molecule):
@@ -238,16 +240,17 @@ def purify(Fo, nocc, tol=1e-11, maxit=200):
     B 66, 155115 (2002)): start from (e_max - Fo)/(e_max - e_min) with
     Gershgorin bounds, then X <- X^2 or 2X - X^2, whichever brings tr X closer
     to nocc.  Stops once tr(X - X^2) < tol and takes one more step.  Device
-    tensors go through the library (pq_purify: its own product, trace and
-    branch kernels); host tensors use the same steps in torch."""
+    tensors go through the library (pq_purify: one fused product, trace and
+    branch launch per step); host tensors use the same steps in torch."""
     m = Fo.shape[0]
     if Fo.device.type == "cuda" and 0 < nocc < m:
         import ctypes
         from . import _lib
         from .engine import _ptr, _stream
-        key = (m, Fo.device)
+        key = (m, int(maxit), Fo.device)
         if key not in _PURIFY_WORK:
-            _PURIFY_WORK[key] = torch.empty(2 * m * m + 4, dtype=torch.float64, device=Fo.device)
+            ld = (m + 31) // 32 * 32
+            _PURIFY_WORK[key] = torch.empty(4 * ld * ld + 2 * int(maxit) + 3, dtype=torch.float64, device=Fo.device)
         Fo = Fo.contiguous()
         P = torch.empty_like(Fo)
         steps, idem = ctypes.c_int(0), ctypes.c_double(0.0)

@@ -105,6 +105,45 @@ def test_jk_vs_dense_einsum(gpu):
     assert np.abs(K.cpu().numpy() - Ko).max() < 1e-10
 
 
+def test_device_cache_reuse_gives_same_jk(gpu):
+    """Buffers of a destroyed basis are cached and handed to the next basis:
+    J/K of a second engine on the same molecule, built from reused blocks,
+    equal the first engine's up to the order of the floating-point atomic
+    additions (a smaller molecule in between takes some of the blocks
+    first); releasing the cache empties it."""
+    import torch
+    from pychem_amd.engine import Engine, cached_bytes, release_cache
+    release_cache()
+    assert cached_bytes() == 0
+    m = _mol(geometry.alkane(4, 0.02, 4000), "6-31G*")
+    N = m.NOrbitals
+    rng = np.random.default_rng(3)
+    A = rng.normal(size=(N, N)) * 0.1
+    D = torch.tensor(A @ A.T / N, device="cuda")
+
+    def jk_once(mol):
+        e = Engine(mol)
+        e.build_eri(1e-8, 1e-6)
+        J, K = e.jk(2.0 * D, D)
+        J, K = J.cpu().numpy(), K.cpu().numpy()
+        e.close()
+        return J, K
+
+    J1, K1 = jk_once(m)
+    held = cached_bytes()
+    assert held > 0
+    small = _mol(geometry.water(), "STO-3G")
+    e = Engine(small)
+    e.build_eri(1e-8)
+    e.jk(torch.eye(small.NOrbitals, dtype=torch.float64, device="cuda"), None)
+    e.close()
+    J2, K2 = jk_once(m)
+    assert np.abs(J1 - J2).max() < 1e-12 and np.abs(K1 - K2).max() < 1e-12
+    assert cached_bytes() >= held
+    release_cache()
+    assert cached_bytes() == 0
+
+
 def test_level_b_surface_matches_oracle(gpu):
     """integrals.one_electron / two_electron (the reference's Methods/integrals.py
     call surface) against the oracle, spherical d included."""

@@ -91,7 +91,7 @@ def test_reference_diis_oscillates_like_oracle(gpu):
     assert np.abs(np.array(r["history"]) - np.array(o["history"])).max() < 1e-7
 
 
-@pytest.mark.parametrize("m,nocc", [(7, 2), (194, 41), (300, 150)])
+@pytest.mark.parametrize("m,nocc", [(7, 2), (33, 10), (194, 41), (300, 150), (384, 81)])
 def test_pq_purify_matches_eigenvector_projector(gpu, m, nocc):
     """Device TC2 purification (pq_purify) against the projector built from
     eigh: a random symmetric matrix with a gap of 0.5 between the occupied

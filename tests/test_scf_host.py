@@ -83,3 +83,18 @@ def test_uhf_doublet_runs_unrestricted(fake_engine):
     o = oscf.run_rhf(m, conv=1e-10, maxiter=200, diis=oscf.PULAY)
     assert r["converged"]
     assert abs(r["energy"] - o["energy"]) < 1e-8
+
+
+def test_nuclear_repulsion_closed_forms():
+    """Z1 Z2 / r for two nuclei; 3 Z^2 / a for an equilateral triangle of
+    side a; one atom has none."""
+    two = structures.Molecule([["O", 8, 0.0, 0.0, 0.0], ["H", 1, 0.0, 0.0, 1.7]], basis="STO-3G", units="BOHR",
+                              multiplicity=2)
+    assert abs(hartree_fock.nuclear_repulsion(two) - 8.0 / 1.7) < 1e-14
+    a = 1.9
+    tri = structures.Molecule([["H", 1, 0.0, 0.0, 0.0], ["H", 1, a, 0.0, 0.0],
+                               ["H", 1, 0.5 * a, 0.5 * a * np.sqrt(3.0), 0.0]], basis="STO-3G", units="BOHR",
+                              multiplicity=2)
+    assert abs(hartree_fock.nuclear_repulsion(tri) - 3.0 / a) < 1e-14
+    one = structures.Molecule([["He", 2, 0.0, 0.0, 0.0]], basis="STO-3G", units="BOHR")
+    assert hartree_fock.nuclear_repulsion(one) == 0.0
