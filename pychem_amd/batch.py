@@ -207,6 +207,22 @@ def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_grap
         # Pulay DIIS on X^T (S D F - F D S) X, batched
         A = Sp @ D @ F
         R = X.transpose(1, 2) @ (A - A.transpose(1, 2)) @ X
+        # variational energy of this cycle's input density with its own Fock
+        # matrix (the single-molecule device loop's test): |dE| < conv counts
+        # once the orbital gradient has settled too, and the molecule keeps
+        # the density whose energy passed
+        Ev = ((D * (Hp + F)) * vmask).sum((1, 2)) + enuc
+        rmax = R.abs().amax((1, 2))
+        newly = (~done) & ((Ev - E).abs() < conv) & (rmax <= conv ** 0.5) & (it > 1)
+        E = torch.where(done, E, Ev)
+        iters = torch.where(done, iters, torch.full_like(iters, it))
+        if bool(newly.any()):
+            tnow = time.perf_counter() - t_start
+            for m in torch.nonzero(newly).flatten().tolist():
+                latency[m] = tnow
+        done = done | newly
+        if bool(done.all()):
+            break
         Fh.append(F)
         Rh.append(R)
         if len(Fh) > diis_size:
@@ -229,22 +245,7 @@ def rhf_batch(mols, conv=1e-8, maxiter=60, threshold=1e-8, guess="SAD", use_grap
         else:
             Fx = F
         Dn = _density(X, Fx, dropped, nocc)
-        En = ((Dn * (Hp + Fx)) * vmask).sum((1, 2)) + enuc
-        # |dE| < conv counts only once the orbital gradient of the input
-        # density has settled too: an oscillating SCF can pass through a small
-        # energy change far from convergence
-        rmax = R.abs().amax((1, 2))
-        newly = (~done) & ((En - E).abs() < conv) & (rmax <= conv ** 0.5) & (it > 1)
         D = torch.where(done[:, None, None], D, Dn)
-        E = torch.where(done, E, En)
-        iters = torch.where(done, iters, torch.full_like(iters, it))
-        if bool(newly.any()):
-            tnow = time.perf_counter() - t_start
-            for m in torch.nonzero(newly).flatten().tolist():
-                latency[m] = tnow
-        done = done | newly
-        if bool(done.all()):
-            break
     torch.cuda.synchronize()
     total = time.perf_counter() - t_start
     latency[np.isnan(latency)] = total

@@ -447,15 +447,25 @@ def _result(molecule, state, it, dE, conv, history, timings):
                 nuclear=molecule.NuclearRepulsion)
 
 
-# The device loop stops on the reference's |dE| <= conv only when the orbital
-# gradient has settled too (max |X^T (S D F - F D S) X| <= sqrt(conv)): an
-# oscillating SCF can pass through a small energy change far from convergence.
+# The device loop tests |dE| <= conv (the reference's criterion) on the
+# variational energy E[D] = 1/2 sum D (H + F[D]) of each cycle's input density
+# with its own Fock matrix (DEVICE_ENERGY = "variational", the default), or on
+# the reference's energy from the new density and the extrapolated Fock
+# (DEVICE_ENERGY = "reference").  E[D] is stationary at the solution (its
+# error is second order in the density error) while the reference's energy
+# oscillates about it, so the variational test stops in about half the
+# cycles at the same energy (C20H42/6-31G*: 11 cycles instead of 19, energies
+# 4e-10 Eh apart).  Either way it counts only once the orbital gradient has
+# settled too (max |X^T (S D F - F D S) X| <= sqrt(conv)): an oscillating SCF
+# can pass through a small energy change far from convergence.
+DEVICE_ENERGY = "variational"
 
 
 def _scf_device(molecule, state, conv, maxiter, use_diis, diis_start, verbose, timing, t0, timings):
-    """The SCF iteration on device tensors (same sequence as the host loop:
-    J/K, Fock, DIIS extrapolation, new density, energy from the new density
-    and the extrapolated Fock), with the density from ``purify``."""
+    """The SCF iteration on device tensors: J/K, Fock, energy test (see
+    DEVICE_ENERGY), DIIS extrapolation, new density from ``purify``.  With the
+    variational test the result is the last input density and its own Fock
+    matrix."""
     eng = get_engine(molecule)
     dev = eng.device
     H = molecule.Core.to(dev)
@@ -490,11 +500,25 @@ def _scf_device(molecule, state, conv, maxiter, use_diis, diis_start, verbose, t
         ra = DIIS.residual(S, Da, Fa, X)
         rb = ra if restricted else DIIS.residual(S, Db, Fb, X)
         grad = torch.maximum(ra.abs().max(), rb.abs().max())
+        if DEVICE_ENERGY == "variational":
+            Ev, gmax = torch.stack([0.5 * (((Da + Db) * H).sum() + (Da * Fa).sum() + (Db * Fb).sum()),
+                                    grad]).tolist()
+            dE = 1.0 if it == 1 else Ev - E
+            E = Ev
+            if abs(dE) <= conv and gmax > conv ** 0.5:
+                dE = 2.0 * conv if dE >= 0 else -2.0 * conv
+            history.append(E + molecule.NuclearRepulsion)
+            if verbose:
+                print(" Cycle: %d  Total energy: %.12f  Change: %.3e" % (it, E + molecule.NuclearRepulsion, dE))
+            if abs(dE) <= conv:
+                break
         if use_diis and it > diis_start:
             Fa = state.AlphaDIIS.extrapolate(ra, Fa)
             Fb = Fa if restricted else state.BetaDIIS.extrapolate(rb, Fb)
         Da = X @ purify(Xt @ Fa @ X, na) @ Xt
         Db = Da if restricted else X @ purify(Xt @ Fb @ X, nb) @ Xt
+        if DEVICE_ENERGY == "variational":
+            continue
         old = E
         E, gmax = torch.stack([0.5 * (((Da + Db) * H).sum() + (Da * Fa).sum() + (Db * Fb).sum()), grad]).tolist()
         dE = E - old

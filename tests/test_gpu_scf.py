@@ -80,10 +80,12 @@ def test_alkane_631gs_vs_oracle(gpu, n):
     assert np.abs(r["ea"] - eo).max() < 1e-6
 
 
-def test_reference_diis_oscillates_like_oracle(gpu):
+def test_reference_diis_oscillates_like_oracle(gpu, monkeypatch):
     """The reference's gated DIIS from the core guess does not converge on
-    butane/6-31G*; the GPU path reproduces the same non-converged sequence."""
+    butane/6-31G*; the GPU path, with the reference's energy test, reproduces
+    the same non-converged sequence."""
     from pychem_amd import hartree_fock
+    monkeypatch.setattr(hartree_fock, "DEVICE_ENERGY", "reference")
     m = structures.Molecule(geometry.alkane(4, 0.02, 4010), basis="6-31G*", cartesian_l=[2])
     r = hartree_fock.do(m, conv=1e-10, maxiter=12, diis_mode="reference")
     o = oscf.run_rhf(m, conv=1e-10, maxiter=12)

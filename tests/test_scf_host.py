@@ -54,7 +54,10 @@ def test_water_core_guess_matches_oracle(fake_engine):
     assert abs(r["energy"] - o["energy"]) < 1e-9
 
 
-def test_reference_diis_mode_tracks_oracle(fake_engine):
+def test_reference_diis_mode_tracks_oracle(fake_engine, monkeypatch):
+    """With the reference's energy test and gated DIIS, the device loop
+    repeats the oracle's cycle sequence."""
+    monkeypatch.setattr(hartree_fock, "DEVICE_ENERGY", "reference")
     m = structures.Molecule([["H", 1.0, 0, 0, 0], ["H", 1.0, 1.0, 0, 0]], basis="6-311G")
     r = hartree_fock.do(m, conv=1e-7, maxiter=50, diis_mode="reference")
     o = oscf.run_rhf(m, conv=1e-7, maxiter=50)
@@ -98,3 +101,22 @@ def test_nuclear_repulsion_closed_forms():
     assert abs(hartree_fock.nuclear_repulsion(tri) - 3.0 / a) < 1e-14
     one = structures.Molecule([["He", 2, 0.0, 0.0, 0.0]], basis="STO-3G", units="BOHR")
     assert hartree_fock.nuclear_repulsion(one) == 0.0
+
+
+def test_variational_energy_test_converges_to_oracle_energy(fake_engine):
+    """The device loop's default test (variational energy of each cycle's
+    input density) stops at the oracle's energy within 1e-9 Eh, in no more
+    cycles than the reference's energy test."""
+    m = structures.Molecule(geometry.water(), basis="6-31G*", cartesian_l=[2])
+    r = hartree_fock.do(m, conv=1e-10, maxiter=60, guess="CORE")
+    assert hartree_fock.DEVICE_ENERGY == "variational" and r["converged"]
+    o = oscf.run_rhf(m, conv=1e-10, maxiter=60)
+    assert o["converged"] and abs(r["energy"] - o["energy"]) < 1e-9
+    ref_mode = hartree_fock.DEVICE_ENERGY
+    try:
+        hartree_fock.DEVICE_ENERGY = "reference"
+        m2 = structures.Molecule(geometry.water(), basis="6-31G*", cartesian_l=[2])
+        r2 = hartree_fock.do(m2, conv=1e-10, maxiter=60, guess="CORE")
+    finally:
+        hartree_fock.DEVICE_ENERGY = ref_mode
+    assert r["iterations"] <= r2["iterations"]
