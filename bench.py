@@ -20,6 +20,7 @@ RHF on the host cores.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--workload c10h22|c4h10|water]
     python bench.py --copy-calibration MIB   # plain device copy, for PMC byte-counter calibration
+    python bench.py --only-rhf --steps 2     # plain RHF solves and kernel timings only (profiling runs)
 """
 import argparse
 import json
@@ -103,7 +104,7 @@ def jk_roofline(mol, reps=10):
         ncd = (lc + 1) * (lc + 2) // 2 * ((ld + 1) * (ld + 2) // 2)
         nq = s["nq64"]
         nbytes = nq * s["ncomp"] * 8 + nq * 4 + nit.value * 16 + nq * nt * ncd * 8
-        out.append(dict(cls=s["cls"], kernel="jk_tile_kernel<%d, %d, %d, %d, %d, %d, double>" % (la, lb, lc, ld, ta, tb),
+        out.append(dict(cls=s["cls"], kernel="jk_tile_kernel<%d, %d, %d, %d, %d, %d>" % (la, lb, lc, ld, ta, tb),
                         nq=nq, items=nit.value, bytes=nbytes, ms=ms, gbs=nbytes / (ms * 1e-3) / 1e9))
     dom = max(out, key=lambda c: c["ms"])
     return out, dom
@@ -264,6 +265,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-size", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
+    ap.add_argument("--only-rhf", action="store_true",
+                    help="plain RHF solves and the kernel timings only (no MP2, mixed, C20H42, batch or CPU legs): "
+                         "the command the per-kernel profiles are taken on")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -298,13 +302,13 @@ def main():
     # MP2 on the same molecule (outside the timed RHF steps)
     rk, _, molk = gpu_solve(factory, keep=True)
     jk_table, jk_dom = jk_roofline(molk)
-    mp2_info = mp2_measure(molk, rk)
+    mp2_info = None if args.only_rhf else mp2_measure(molk, rk)
     cls_alone = eri_class_alone_ms(molk.Engine)
     molk.Engine.close()
     # mixed precision: Schwarz-split FP32 ERIs below a bound of 1e-6
     mixed = []
     t0m = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(0 if args.only_rhf else args.steps):
         mixed.append(gpu_solve(factory, fp32_threshold=1e-6))
     torch.cuda.synchronize()
     mixed_s = (time.perf_counter() - t0m) / args.steps
@@ -359,18 +363,20 @@ def main():
         "eri_roofline": eri_roofline,
         "per_class": per_class,
         "mp2": mp2_info,
-        "mixed": {"time_to_solution_s": mixed_s, "speedup_vs_fp64": (ms_per_step / 1e3) / mixed_s,
-                  "energy_error_vs_fp64": mixed[-1][0]["energy"] - r0["energy"],
-                  "scf_iterations": mixed[-1][0]["iterations"],
-                  "fp32_quartets": sum(s["nq32"] for s in mixed[-1][1]),
-                  "fp64_quartets": sum(s["nq64"] for s in mixed[-1][1]),
-                  "eri_pass_ms": mixed[-1][0]["timings"]["eri_pass_ms"],
-                  "jk_ms": float(np.mean(mixed[-1][0]["timings"]["jk_ms"]))},
     }
-    out["headline_c20"] = headline_c20(args.steps)
-    if not args.no_batch:
+    if mixed:
+        out["mixed"] = {"time_to_solution_s": mixed_s, "speedup_vs_fp64": (ms_per_step / 1e3) / mixed_s,
+                        "energy_error_vs_fp64": mixed[-1][0]["energy"] - r0["energy"],
+                        "scf_iterations": mixed[-1][0]["iterations"],
+                        "fp32_quartets": sum(s["nq32"] for s in mixed[-1][1]),
+                        "fp64_quartets": sum(s["nq64"] for s in mixed[-1][1]),
+                        "eri_pass_ms": mixed[-1][0]["timings"]["eri_pass_ms"],
+                        "jk_ms": float(np.mean(mixed[-1][0]["timings"]["jk_ms"]))}
+    if not args.only_rhf:
+        out["headline_c20"] = headline_c20(args.steps)
+    if not args.no_batch and not args.only_rhf:
         out["batch"] = batch_measure(args.batch_size, args.steps)
-    if not args.no_cpu:
+    if not args.no_cpu and not args.only_rhf:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         cb = cpu_baseline(args.workload, threads)
         out["cpu_baseline"] = {"value": cb["nquartets"] / cb["seconds"], "unit": out["unit"], "cores": threads,
