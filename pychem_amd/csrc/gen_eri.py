@@ -389,7 +389,8 @@ def main():
             jsrc.append("void jk_run_%s_%s(const JkArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
             jsrc.append("{ hipLaunchKernelGGL((jk_kernel<%d, %d, %d, %d, %s>), dim3((a.nq + 127) / 128), dim3(128), 0, st, a); }" % (la, lb, lc, ld, rt))
             jsrc.append("void scatter_run_%s_%s(const ScatterArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
-            jsrc.append("{ hipLaunchKernelGGL((scatter_kernel<%d, %d, %d, %d, %s>), dim3((a.nq + 127) / 128), dim3(128), 0, st, a); }" % (la, lb, lc, ld, rt))
+            jsrc.append("{ hipLaunchKernelGGL((scatter_kernel<%d, %d, %d, %d, %s>), dim3((unsigned)((a.nq * %dLL + 255) / 256)), "
+                        "dim3(256), 0, st, a); }" % (la, lb, lc, ld, rt, ncomp))
         _write_if_changed(os.path.join(gdir, "cls_%s.hip" % tag), "\n".join(src) + "\n")
         ta, tb, nt = jk_tile(la, lb, lc, ld)
         jsrc.append("void jktile_run_%s(const JkTileArgs& a, int block, hipStream_t st)" % tag)

@@ -332,36 +332,47 @@ __global__ void __launch_bounds__(256) jk_tile_kernel(const JkTileArgs a)
 }
 
 // Dense scatter with 8-fold permutational symmetry (Methods/hartree_fock.py:314-325)
-// into a slab of third indices l in [l0, l0 + lc): out[m][n][l - l0][s].
+// into a slab of third indices l in [l0, l0 + lc): out[m][n][l - l0][s].  One
+// thread per (component, quartet), component-major, so that consecutive
+// threads read consecutive values: a single high-l quartet (an atom's
+// (dd|dd)) spreads over its components instead of one thread writing them
+// all.
 template <int LA, int LB, int LC, int LD, typename real>
-__global__ void __launch_bounds__(128) scatter_kernel(const ScatterArgs<real> a)
+__global__ void __launch_bounds__(256) scatter_kernel(const ScatterArgs<real> a)
 {
-    constexpr int N1 = (LA + 1) * (LA + 2) / 2, N2 = (LB + 1) * (LB + 2) / 2;
+    constexpr int N2 = (LB + 1) * (LB + 2) / 2;
     constexpr int N3 = (LC + 1) * (LC + 2) / 2, N4 = (LD + 1) * (LD + 2) / 2;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.nq) return;
+    constexpr int N1 = (LA + 1) * (LA + 2) / 2, NC = N1 * N2 * N3 * N4;
+    const long long nq = a.nq;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= nq * NC) return;
+    const int comp = (int)(t / nq);
+    const long long q = t - comp * nq;
+    const int id = comp % N4, ic = (comp / N4) % N3, ib = (comp / (N3 * N4)) % N2, ia = comp / (N2 * N3 * N4);
     const int2 pr = a.quartets[q];
     const PairHdr bh = a.pairs[pr.x], kh = a.pairs[pr.y];
-    const long long o1 = a.cart_off[bh.s1], o2 = a.cart_off[bh.s2];
-    const long long o3 = a.cart_off[kh.s1], o4 = a.cart_off[kh.s2];
-    const long long n = a.n, nq = a.nq, l0 = a.l0, lc = a.lc;
-    auto put = [&](long long m, long long nn, long long l, long long s, double v) {
-        const long long t = l - l0;
-        if (t >= 0 && t < lc) a.out[((m * n + nn) * lc + t) * n + s] = v;
+    // Positions reached from two components (a pair of one shell with
+    // itself, a quartet of one pair with itself) hold values equal up to
+    // rounding; one canonical component writes them, so that the result
+    // does not depend on the order of the threads.  The symmetric images
+    // of the kept components still cover every position.
+    if (bh.s1 == bh.s2 && ia < ib) return;
+    if (kh.s1 == kh.s2 && ic < id) return;
+    if (pr.x == pr.y && ia * N2 + ib < ic * N4 + id) return;
+    const long long n = a.n, l0 = a.l0, lc = a.lc;
+    const long long m = a.cart_off[bh.s1] + ia, nn = a.cart_off[bh.s2] + ib;
+    const long long l = a.cart_off[kh.s1] + ic, s = a.cart_off[kh.s2] + id;
+    const double v = (double)a.vals[(size_t)comp * nq + q];
+    auto put = [&](long long i, long long j, long long k, long long r) {
+        const long long u = k - l0;
+        if (u >= 0 && u < lc) a.out[((i * n + j) * lc + u) * n + r] = v;
     };
-    for (int ia = 0; ia < N1; ++ia)
-        for (int ib = 0; ib < N2; ++ib)
-            for (int ic = 0; ic < N3; ++ic)
-                for (int id = 0; id < N4; ++id) {
-                    const double v = (double)a.vals[(size_t)(((ia * N2 + ib) * N3 + ic) * N4 + id) * nq + q];
-                    const long long m = o1 + ia, nn = o2 + ib, l = o3 + ic, s = o4 + id;
-                    put(m, nn, l, s, v);
-                    put(nn, m, l, s, v);
-                    put(m, nn, s, l, v);
-                    put(nn, m, s, l, v);
-                    put(l, s, m, nn, v);
-                    put(l, s, nn, m, v);
-                    put(s, l, m, nn, v);
-                    put(s, l, nn, m, v);
-                }
+    put(m, nn, l, s);
+    put(nn, m, l, s);
+    put(m, nn, s, l);
+    put(nn, m, s, l);
+    put(l, s, m, nn);
+    put(l, s, nn, m);
+    put(s, l, m, nn);
+    put(s, l, nn, m);
 }

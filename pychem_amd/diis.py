@@ -21,6 +21,7 @@ class DIIS:
         self.focks = []
         self.residuals = []
         self.matrix = None
+        self.gram = np.zeros((0, 0))
         self.error = 1.0
 
     @staticmethod
@@ -40,49 +41,67 @@ class DIIS:
         return float((a * b).sum())
 
     @staticmethod
-    def _dots(a, bs):
-        """<a, b> for every b, with one device-to-host transfer for tensors."""
-        if hasattr(a, "device") and getattr(a.device, "type", "cpu") != "cpu":
-            import torch
-            return torch.stack([(a * b).sum() for b in bs]).tolist()
-        return [float((a * b).sum()) for b in bs]
+    def _bordered(g):
+        """The bordered B matrix [[G, -1], [-1, 0]] of the C1 DIIS system."""
+        k = len(g)
+        B = np.full((k + 1, k + 1), -1.0)
+        B[:k, :k] = g
+        B[k, k] = 0.0
+        return B
 
     @staticmethod
     def _cond(m):
         ev = np.abs(np.linalg.eigvals(m))
         return np.inf if ev.min() == 0.0 else ev.max() / ev.min()
 
-    def extrapolate(self, res, F):
-        if hasattr(res, "device") and getattr(res.device, "type", "cpu") != "cpu":
-            import torch
-            self.error, amax = torch.stack([res.max(), res.abs().max()]).tolist()
-        else:
-            self.error, amax = float(res.max()), float(abs(res).max())
+    @staticmethod
+    def _is_tensor(x):
+        return hasattr(x, "device") and hasattr(x, "abs")
+
+    def stats(self, res):
+        """The scalars extrapolate() needs for ``res``: max(res), max|res| and
+        <res, r> for every stored residual r and for res itself.  Tensor
+        residuals give a list of 0-d tensors, so that a caller can fetch them
+        together with its own scalars in one transfer; numpy residuals give
+        floats."""
+        if self._is_tensor(res):
+            return [res.max(), res.abs().max()] + [(res * r).sum() for r in self.residuals] + [(res * res).sum()]
+        return [float(res.max()), float(abs(res).max())] + [self._dot(res, r) for r in self.residuals] + \
+            [self._dot(res, res)]
+
+    def extrapolate(self, res, F, pre=None):
+        """DIIS-extrapolated Fock matrix after adding (res, F); ``pre`` = the
+        host values of stats(res) when the caller fetched them already."""
+        if pre is None:
+            st = self.stats(res)
+            if self._is_tensor(res):
+                import torch
+                st = torch.stack(st).tolist()
+            pre = st
+        self.error, amax = float(pre[0]), float(pre[1])
+        dots = np.asarray(pre[2:], dtype=float)
         if abs(self.error) >= abs(self.threshold):
             return F
         if amax < 1e-14:
             return F          # converged: nothing to extrapolate
+        # Gram matrix of the stored residuals, grown by the new row
+        k = len(self.residuals)
+        g = np.empty((k + 1, k + 1))
+        g[:k, :k] = self.gram
+        g[k, :] = dots
+        g[:, k] = dots
+        self.gram = g
         self.residuals.append(res)
         self.focks.append(F)
         if len(self.residuals) < 2:
             return F
-        size = len(self.residuals) + 1
-        B = np.full((size, size), -1.0)
-        B[-1, -1] = 0.0
-        if self.matrix is None:
-            for i, r1 in enumerate(self.residuals):
-                B[i, :size - 1] = self._dots(r1, self.residuals)
-        else:
-            B[:size - 1, :size - 1] = self.matrix
-            new = self._dots(self.residuals[-1], self.residuals)
-            B[:len(new), size - 2] = new
-            B[size - 2, :len(new)] = new
-        self.matrix = B
+        self.matrix = self._bordered(self.gram)
         while (self.max_size is not None and len(self.residuals) > self.max_size) or \
                 self._cond(self.matrix) > self.max_condition:
             if len(self.residuals) == 1:
                 break
-            self.matrix = np.delete(np.delete(self.matrix, 0, 0), 0, 1)
+            self.gram = self.gram[1:, 1:]
+            self.matrix = self._bordered(self.gram)
             self.residuals.pop(0)
             self.focks.pop(0)
         rhs = np.zeros(len(self.matrix))

@@ -51,6 +51,10 @@ def _atomic_density(label, Z, basis, cartesian_l, maxiter=60, conv=1e-8):
             i = j + 1
         return occ
 
+    # J_ab = sum_cd (ab|cd) D_cd and K_ad = sum_bc (ab|cd) D_cb as two
+    # matrix-vector products over the reshaped tensor
+    eri_j = eri.reshape(n * n, n * n)
+    eri_k = np.ascontiguousarray(eri.transpose(0, 3, 2, 1)).reshape(n * n, n * n)
     F = H.copy()
     D = np.zeros((n, n))
     E_old = 0.0
@@ -60,8 +64,8 @@ def _atomic_density(label, Z, basis, cartesian_l, maxiter=60, conv=1e-8):
         occ = occupations(e)
         D_new = (C * occ) @ C.T
         D = D_new if _ == 0 else 0.5 * (D + D_new)
-        J = np.einsum("abcd,cd->ab", eri, D)
-        K = np.einsum("abcd,cb->ad", eri, D)
+        J = (eri_j @ D.reshape(-1)).reshape(n, n)
+        K = (eri_k @ D.reshape(-1)).reshape(n, n)
         F = H + J - 0.5 * K
         E = 0.5 * np.sum(D * (H + F))
         if abs(E - E_old) < conv:

@@ -500,9 +500,18 @@ def _scf_device(molecule, state, conv, maxiter, use_diis, diis_start, verbose, t
         ra = DIIS.residual(S, Da, Fa, X)
         rb = ra if restricted else DIIS.residual(S, Db, Fb, X)
         grad = torch.maximum(ra.abs().max(), rb.abs().max())
+        diis_now = use_diis and it > diis_start
         if DEVICE_ENERGY == "variational":
-            Ev, gmax = torch.stack([0.5 * (((Da + Db) * H).sum() + (Da * Fa).sum() + (Db * Fb).sum()),
-                                    grad]).tolist()
+            # energy, gradient and the DIIS scalars of both spins in one
+            # device-to-host transfer
+            parts = [0.5 * (((Da + Db) * H).sum() + (Da * Fa).sum() + (Db * Fb).sum()), grad]
+            na_st = 0
+            if diis_now:
+                sa = state.AlphaDIIS.stats(ra)
+                na_st = len(sa)
+                parts += sa + ([] if restricted else state.BetaDIIS.stats(rb))
+            vals = torch.stack(parts).tolist()
+            Ev, gmax = vals[0], vals[1]
             dE = 1.0 if it == 1 else Ev - E
             E = Ev
             if abs(dE) <= conv and gmax > conv ** 0.5:
@@ -512,7 +521,10 @@ def _scf_device(molecule, state, conv, maxiter, use_diis, diis_start, verbose, t
                 print(" Cycle: %d  Total energy: %.12f  Change: %.3e" % (it, E + molecule.NuclearRepulsion, dE))
             if abs(dE) <= conv:
                 break
-        if use_diis and it > diis_start:
+            if diis_now:
+                Fa = state.AlphaDIIS.extrapolate(ra, Fa, pre=vals[2:2 + na_st])
+                Fb = Fa if restricted else state.BetaDIIS.extrapolate(rb, Fb, pre=vals[2 + na_st:])
+        elif diis_now:
             Fa = state.AlphaDIIS.extrapolate(ra, Fa)
             Fb = Fa if restricted else state.BetaDIIS.extrapolate(rb, Fb)
         Da = X @ purify(Xt @ Fa @ X, na) @ Xt
