@@ -313,27 +313,70 @@ def _as_host(M):
     return torch.as_tensor(np.asarray(M, dtype=float))
 
 
+def mo_file_names(basis_set, state_index=0, directory="."):
+    """(alpha, beta) MO file names for a basis set and state: the reference's
+    `<basis>_<state>.alpha_MOs` / `.beta_MOs`, basis without punctuation
+    (Methods/hartree_fock.py:152-153, Util/structures.py:223-224)."""
+    b = structures.remove_punctuation(basis_set)
+    return (os.path.join(directory, "%s_%s.alpha_MOs" % (b, state_index)),
+            os.path.join(directory, "%s_%s.beta_MOs" % (b, state_index)))
+
+
+def dump_mos(molecule, basis_set, state_index=0, directory="."):
+    """Write the converged alpha and beta MO coefficients (rows: basis
+    functions, columns: orbitals) as numpy text, the reference's Dump_MOs
+    output (Methods/hartree_fock.py:151-153).  Returns the file names."""
+    state = molecule.State
+    files = mo_file_names(basis_set, state_index, directory)
+    for path, C in zip(files, (state.Alpha.MOs, state.Beta.MOs)):
+        np.savetxt(path, np.asarray(C.numpy() if isinstance(C, torch.Tensor) else C, dtype=float))
+    return files
+
+
+def read_mos(molecule, alpha_file, beta_file):
+    """MO coefficients from the reference's text files (the READ guess,
+    Methods/hartree_fock.py:35-50): both sets must have one row per basis
+    function."""
+    mos = []
+    for path in (alpha_file, beta_file):
+        C = np.loadtxt(path, ndmin=2)
+        if C.shape[0] != molecule.NOrbitals:
+            raise ValueError("Incorrect MOs supplied in %s: %d rows for %d basis functions - check basis set and "
+                             "supply both alpha and beta sets" % (path, C.shape[0], molecule.NOrbitals))
+        mos.append(torch.as_tensor(C, dtype=F64))
+    return mos[0], mos[1]
+
+
 def do(molecule, conv=structures.ENERGY_CONVERGENCE, maxiter=25, use_diis=True, diis_start=1,
        guess="CORE", threshold=structures.INTEGRAL_THRESHOLD, fp32_threshold=0.0, initial_run=True,
-       verbose=False, timing=False, diis_mode="pulay", loop="device", nets=None):
+       verbose=False, timing=False, diis_mode="pulay", loop="device", nets=None, mo_files=None, dump=None):
     """SCF for the ground state (Methods/hartree_fock.py:23-154).
 
     ``guess`` is "CORE" (the reference default), "SAD", "MLP" (the learned
     guess of mlp_guess, ``nets`` or the shipped networks; SAD where they do
-    not cover the molecule), or an (N x N) density
+    not cover the molecule), "READ" (MO coefficients from the text files
+    ``mo_files`` = (alpha, beta), see mo_file_names), or an (N x N) density
     matrix per spin given as a tuple (Da, Db), or a single density used for
     both.  ``diis_mode`` is "pulay" (default) or "reference" (the reference's
     gated DIIS).  ``loop`` is "device" (default: the iteration stays on the
     GPU, densities by purification) or "host" (the reference's per-iteration
-    diagonalisation on the host).  Returns a dict with the energy, iteration count,
-    convergence flag, MOs, orbital energies, densities (numpy) and per-stage
-    timings.
+    diagonalisation on the host).  ``dump`` = (basis_set, state_index[,
+    directory]) writes the final MOs like the reference's Dump_MOs.  Returns a
+    dict with the energy, iteration count, convergence flag, MOs, orbital
+    energies, densities (numpy) and per-stage timings.
     """
     if loop not in ("device", "host"):
         raise ValueError("loop must be 'device' or 'host'")
+    if isinstance(guess, str) and guess == "READ":
+        if not mo_files or len(mo_files) != 2:
+            raise ValueError("guess='READ' needs mo_files=(alpha_file, beta_file)")
+        guess = ("MOS",) + tuple(read_mos(molecule, *mo_files))
     with host_threads(molecule.NOrbitals):
-        return _do(molecule, conv, maxiter, use_diis, diis_start, guess, threshold, fp32_threshold,
-                   initial_run, verbose, timing, diis_mode, loop, nets)
+        r = _do(molecule, conv, maxiter, use_diis, diis_start, guess, threshold, fp32_threshold,
+                initial_run, verbose, timing, diis_mode, loop, nets)
+    if dump is not None:
+        r["mo_files"] = dump_mos(molecule, *dump)
+    return r
 
 
 def _do(molecule, conv, maxiter, use_diis, diis_start, guess, threshold, fp32_threshold, initial_run,
@@ -381,6 +424,10 @@ def _do(molecule, conv, maxiter, use_diis, diis_start, guess, threshold, fp32_th
         if guess != "CORE":
             raise ValueError("unknown guess %r" % guess)
         make_MOs(molecule, state)
+        make_density_matrices(molecule, state)
+    elif isinstance(guess, tuple) and len(guess) == 3 and isinstance(guess[0], str) and guess[0] == "MOS":
+        # READ: densities of the first NOcc columns of the given MOs
+        state.Alpha.MOs, state.Beta.MOs = guess[1], guess[2]
         make_density_matrices(molecule, state)
     else:
         Da, Db = guess if isinstance(guess, tuple) else (guess, guess)

@@ -281,7 +281,18 @@ def read_input(path):
             "DIISSize": get("DIIS_Size", 15),
             "DIISStart": get("DIIS_Start", 1),
             "Guess": get("SCF_Guess", "CORE"),
+            "DumpMOs": get("Dump_MOs", False) is True,
         }
+        if settings["Guess"] == "READ":
+            # Util/structures.py:217-227: one file pair per listed state
+            if "MO_READ_STATE" not in raw or "MO_READ_BASIS" not in raw:
+                raise ValueError("must give the files to read alpha and beta MOs from, using the keywords "
+                                 "MO_Read_Basis and MO_Read_State")
+            states = get("MO_Read_State")
+            states = [str(i) for i in states] if isinstance(states, (list, tuple)) else str(states)
+            mo_basis = remove_punctuation(get("MO_Read_Basis"))
+            settings["MOFiles"] = [(mo_basis + "_" + st + ".alpha_MOs", mo_basis + "_" + st + ".beta_MOs")
+                                   for st in states]
         mol = Molecule(ast.literal_eval(raw["COORDS"]), basis=settings["BasisSets"][0],
                        charge=get("Charge", 0), multiplicity=get("Multiplicity", 1),
                        cartesian_l=get("Cartesian_L", []) or [], units=units)

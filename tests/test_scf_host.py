@@ -120,3 +120,24 @@ def test_variational_energy_test_converges_to_oracle_energy(fake_engine):
     finally:
         hartree_fock.DEVICE_ENERGY = ref_mode
     assert r["iterations"] <= r2["iterations"]
+
+
+def test_mo_dump_and_read_guess_round_trip(fake_engine, tmp_path):
+    """dump writes the reference's Dump_MOs files (numpy text, one row per
+    basis function, '%.18e' fields: an exact round trip); the READ guess
+    from them restarts at the converged energy within two cycles; a file
+    for another basis is refused."""
+    m = structures.Molecule(geometry.water(), basis="STO-3G")
+    r = hartree_fock.do(m, conv=1e-10, maxiter=60, guess="CORE", dump=("STO-3G", 0, str(tmp_path)))
+    fa, fb = r["mo_files"]
+    assert fa.endswith("STO3G_0.alpha_MOs") and fb.endswith("STO3G_0.beta_MOs")
+    assert np.array_equal(np.loadtxt(fa), r["Ca"]) and np.array_equal(np.loadtxt(fb), r["Cb"])
+    first = open(fa).readline().split()
+    assert len(first) == m.NOrbitals
+    assert all(len(x.lstrip("-").split("e")[0]) == 20 for x in first)
+    m2 = structures.Molecule(geometry.water(), basis="STO-3G")
+    r2 = hartree_fock.do(m2, conv=1e-10, maxiter=60, guess="READ", mo_files=(fa, fb))
+    assert r2["converged"] and r2["iterations"] <= 2 and abs(r2["energy"] - r["energy"]) < 1e-10
+    m3 = structures.Molecule(geometry.water(), basis="6-31G")
+    with pytest.raises(ValueError):
+        hartree_fock.do(m3, conv=1e-10, maxiter=5, guess="READ", mo_files=(fa, fb))

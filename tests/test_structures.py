@@ -3,6 +3,7 @@ import math
 import os
 
 import numpy as np
+import pytest
 
 from pychem_amd import geometry, structures
 
@@ -51,6 +52,22 @@ def test_read_input_reference_format(tmp_path):
     assert name == "example5" and mol.NOrbitals == 2
     assert st["Reference"] == "RHF" and st["MaxIter"] == 50
     assert abs(mol.Atoms[1].Coordinates[0] - 1.398397232) < 1e-12   # BOHR units
+
+
+def test_read_input_mo_file_keys(tmp_path):
+    """SCF_Guess = READ names one (alpha, beta) file pair per listed state
+    from MO_Read_Basis without punctuation (Util/structures.py:217-227);
+    Dump_MOs is a boolean (Util/structures.py:170-174)."""
+    text = open(os.path.join(ROOT, "tests", "golden", "example5.inp")).read()
+    text += "SCF_Guess = 'READ'\nMO_Read_Basis = '6-31G*'\nMO_Read_State = [0, 1]\nDump_MOs = True\n"
+    p = tmp_path / "ex.inp"
+    p.write_text(text)
+    (_, _, st), = structures.read_input(str(p))
+    assert st["Guess"] == "READ" and st["DumpMOs"] is True
+    assert st["MOFiles"] == [("631GS_0.alpha_MOs", "631GS_0.beta_MOs"), ("631GS_1.alpha_MOs", "631GS_1.beta_MOs")]
+    p.write_text(text.replace("MO_Read_State = [0, 1]\n", ""))
+    with pytest.raises(ValueError):
+        structures.read_input(str(p))
 
 
 def test_shell_table_offsets():
