@@ -282,11 +282,30 @@ __device__ __forceinline__ void jk_tile_item(const JkTileArgs& a, int item, doub
     const double* __restrict__ Dk = a.Dk + mi.base;
     double* const Jm = a.J + mi.base;
     double* const Km = a.K + mi.base;
-    for (int i = tid; i < NR * n; i += nt) {
-        const int r = i / n, c = i - r * n;
-        const int row = r < TA ? o1 + a0 + r : o2 + b0 + (r - TA);
-        drows[i] = Dk[(size_t)row * n + c];
-        krows[i] = 0.0;
+    // stage the Dk rows eight elements a thread at a time, all eight loads
+    // issued before the first LDS write (the index is clamped into range so
+    // that no branch separates them; out-of-range slots are not written)
+    {
+        constexpr int SU = 8;
+        const int tot = NR * n;
+        for (int i0 = tid; i0 < tot; i0 += SU * nt) {
+            double v[SU];
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int i = min(i0 + u * nt, tot - 1);
+                const int r = i / n, c = i - r * n;
+                const int row = r < TA ? o1 + a0 + r : o2 + b0 + (r - TA);
+                v[u] = Dk[(size_t)row * n + c];
+            }
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int i = i0 + u * nt;
+                if (i < tot) {
+                    drows[i] = v[u];
+                    krows[i] = 0.0;
+                }
+            }
+        }
     }
     double d12[TA][TB], j12[TA][TB];
 #pragma unroll

@@ -841,20 +841,22 @@ static ListBras list_bras(EriClassStore& s)
 
 // J/K work items of a class on the device (kept with the FP64 store): per bra
 // pair, its FP64 kets then its FP32 kets, cut into pieces of at most `run`
-// quartets.  Automatic length: every item stages and flushes 2 (TA + TB) rows
-// of n elements, so its ERI values (run * ncomp) should outweigh that four
-// times over; capped so that the class keeps at least 512 items, within
-// 64..2048.
+// quartets.  Automatic length: every item stages and flushes TA + TB rows of
+// Dk and K' of n elements each, so its ERI values (run * ncomp) should
+// outweigh one of those row sets sixteen times over, within 64..1024.  Timed
+// against the autotuner on C10H22 and C20H42/6-31G*: the staging and flush
+// per item cost more than the parallelism of more, shorter items gains (the
+// earlier rule, four times over and at least 512 items per class, was 12 %
+// slower on C10H22 and 25 % on C20H42 than the tuned lengths; this one lands
+// on the tuned length for most classes of C10H22).
 static long long item_run(const pq_basis* b, int c)
 {
     const EriClassStore& s = b->store64[c];
-    const long long nq = (long long)b->store64[c].nq + b->store32[c].nq;
     long long run = s.jk_run;
     if (run <= 0) {
         const long long rows = jk_tile_table[c][3] / 2;
-        run = (4LL * rows * b->nmax + s.ncomp - 1) / s.ncomp;
-        run = std::min(run, std::max(64LL, nq / 512));
-        run = std::max(64LL, std::min(2048LL, (run + 63) / 64 * 64));
+        run = (16LL * rows * b->nmax + s.ncomp - 1) / s.ncomp;
+        run = std::max(64LL, std::min(1024LL, (run + 63) / 64 * 64));
     }
     return run;
 }
