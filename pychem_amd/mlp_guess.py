@@ -509,10 +509,11 @@ def assemble_density(mol, blocks, groups=None, device="cuda"):
     return D
 
 
-def predict_density(mol, nets, device="cuda"):
+def predict_density(mol, nets, device="cuda", groups=None):
     """Alpha-density guess from the networks' local-frame blocks (pair types
-    without a network stay zero)."""
-    groups = pair_data(mol)
+    without a network stay zero); ``groups`` = pair_data(mol) when the caller
+    has it already."""
+    groups = pair_data(mol) if groups is None else groups
     blocks = {}
     for key, (a, b, F, R) in groups.items():
         net = nets.get(key)
@@ -536,10 +537,10 @@ def purify(D, S, nalpha, rounds=2):
     return D
 
 
-def guess_density(mol, nets, rounds=2):
+def guess_density(mol, nets, rounds=2, groups=None):
     """Learned guess for an SCF: predicted alpha density, purified against
     the overlap on the device (needs mol.Overlap)."""
-    D = predict_density(mol, nets)
+    D = predict_density(mol, nets, groups=groups)
     S = mol.Overlap.to(D.device) if isinstance(mol.Overlap, torch.Tensor) else torch.as_tensor(mol.Overlap, device=D.device)
     return purify(D, S, mol.NAlphaElectrons, rounds)
 
@@ -590,10 +591,10 @@ def learned_guess(mol, nets=None):
     nets = default_nets() if nets is None else nets
     if not nets or mol.Basis != "631GS" or 2 not in mol.CartesianL:
         return None
-    types = set(pair_data(mol).keys())
-    if not types <= set(nets):
+    groups = pair_data(mol)
+    if not set(groups) <= set(nets):
         return None
-    return guess_density(mol, nets)
+    return guess_density(mol, nets, groups=groups)
 
 
 def training_molecules(count, seed=5, sigma=0.05):
