@@ -196,6 +196,20 @@ int pq_mp2_ovov(pq_basis* b, int o1, int v1, const double* Co1, const double* Cv
 int pq_purify(const double* F_dev, int m, int nocc, double* P_dev, double* work_dev, double tol, int max_steps,
               int* steps_out, double* idem_out, void* stream);
 
+/* Scalars of one device SCF cycle (no reference counterpart: the
+ * reference forms them on the host, the energy in Methods/hartree_fock.py:
+ * 188-201 and the DIIS overlaps in Methods/hf_extensions/diis.py:62-75):
+ * out_host[i] = sum_e x[i][e] y[i][e] for i < npair (<= 24), then max and
+ * max-abs of each of the nr (<= 2) arrays r[j], all over n elements; two
+ * launches, one copy back.  work: pq_scf_dots_work(n, npair + 2 nr)
+ * doubles.  Pointer arrays are host arrays of device pointers. */
+long long pq_scf_dots_work(long long n, int nout);
+int pq_scf_dots(const double* const* x_dev, const double* const* y_dev, int npair, const double* const* r_dev,
+                int nr, long long n, double* work_dev, double* out_host, void* stream);
+/* out = sum_i c[i] m[i] over n elements, k <= 16 (the DIIS extrapolation of
+ * Methods/hf_extensions/diis.py:108-118 in one launch). */
+int pq_lincomb(const double* const* m_dev, const double* c_host, int k, long long n, double* out_dev, void* stream);
+
 /* MP2 energy sum over ovov [o1][v1][o2][v2] with D = e_i + e_j - e_a - e_b:
  * mode 0 closed shell sum v(2v - (ib|ja))/D; mode 1 same spin
  * 1/4 sum ((ia|jb) - (ib|ja))^2 / D; mode 2 opposite spin sum v^2 / D

@@ -1611,6 +1611,54 @@ extern "C" int pq_purify(const double* F, int m, int nocc, double* P, double* wo
     return 0;
 }
 
+// Scalars of one SCF cycle (energy terms, DIIS overlaps, residual maxima)
+// in two launches and one device-to-host copy.
+extern "C" long long pq_scf_dots_work(long long n, int nout) { return scf_dots_work(n, nout); }
+
+extern "C" int pq_scf_dots(const double* const* x, const double* const* y, int npair, const double* const* r, int nr,
+                           long long n, double* work, double* out_host, void* stream)
+{
+    if (npair < 0 || npair > SCF_MAXDOT || nr < 0 || nr > 2 || n <= 0 || !work || !out_host)
+        return fail("pq_scf_dots: bad argument (npair <= 24, nr <= 2, n > 0)");
+    if (npair + nr == 0) return 0;
+    ScfDotArgs a = {};
+    for (int i = 0; i < npair; ++i) {
+        if (!x[i] || !y[i]) return fail("pq_scf_dots: null operand");
+        a.x[i] = x[i];
+        a.y[i] = y[i];
+    }
+    for (int j = 0; j < nr; ++j) {
+        if (!r[j]) return fail("pq_scf_dots: null operand");
+        a.r[j] = r[j];
+    }
+    a.npair = npair;
+    a.nr = nr;
+    a.n = n;
+    hipStream_t st = (hipStream_t)stream;
+    if (scf_dots(a, work, st)) return fail("pq_scf_dots: launch failed");
+    const int nout = npair + 2 * nr;
+    HIPCHK(hipMemcpyAsync(out_host, work + (size_t)scf_dots_blocks(n) * nout, sizeof(double) * nout,
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// out = sum_i c_i m_i (k <= 16 device matrices of n elements).
+extern "C" int pq_lincomb(const double* const* m, const double* c, int k, long long n, double* out, void* stream)
+{
+    if (k < 1 || k > SCF_MAXCOMB || n <= 0 || !m || !c || !out) return fail("pq_lincomb: bad argument (1 <= k <= 16)");
+    LinCombArgs a = {};
+    for (int i = 0; i < k; ++i) {
+        if (!m[i]) return fail("pq_lincomb: null matrix");
+        a.m[i] = m[i];
+        a.c[i] = c[i];
+    }
+    a.k = k;
+    a.n = n;
+    if (lincomb(a, out, (hipStream_t)stream)) return fail("pq_lincomb: launch failed");
+    return 0;
+}
+
 extern "C" int pq_mp2_energy(const double* ovov, const double* eo1, const double* ev1, const double* eo2,
                              const double* ev2, int o1, int v1, int o2, int v2, int mode, double* out_host,
                              void* stream)

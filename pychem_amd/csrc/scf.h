@@ -35,3 +35,33 @@ int tc2_steps(const Tc2Work& w, int nocc, double tol, int k0, int steps, hipStre
 // P = X_n symmetrised, n - 1 the recorded last step (or queued - 1 when no
 // step recorded one), X_n from step n - 1's pair and traces.
 int tc2_finish(const Tc2Work& w, int nocc, int queued, double* P, hipStream_t st);
+
+// Scalars of one SCF cycle: dot products sum_e x_i[e] y_i[e] of npair
+// pairs, then max and max-abs of nr arrays (out: npair + 2 nr values, in
+// that order), over n elements.  work: scf_dots_work(n, nout) doubles; the
+// results land at work + scf_dots_blocks(n) * nout.
+#define SCF_MAXDOT 24
+struct ScfDotArgs {
+    const double* x[SCF_MAXDOT];
+    const double* y[SCF_MAXDOT];
+    const double* r[2];
+    int npair, nr;
+    long long n;
+};
+inline int scf_dots_blocks(long long n)
+{
+    const long long b = (n + 2047) / 2048;
+    return (int)(b < 1 ? 1 : (b > 256 ? 256 : b));
+}
+inline long long scf_dots_work(long long n, int nout) { return (long long)scf_dots_blocks(n) * nout + nout; }
+int scf_dots(const ScfDotArgs& a, double* work, hipStream_t st);
+
+// out = sum_i c_i m_i (k <= SCF_MAXCOMB matrices of n elements).
+#define SCF_MAXCOMB 16
+struct LinCombArgs {
+    const double* m[SCF_MAXCOMB];
+    double c[SCF_MAXCOMB];
+    int k;
+    long long n;
+};
+int lincomb(const LinCombArgs& a, double* out, hipStream_t st);

@@ -24,6 +24,8 @@
 // latency-bound: no barrier inside the k loop, one launch per step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "scf.h"
 
 typedef double tc2_d4 __attribute__((ext_vector_type(4)));
@@ -342,5 +344,103 @@ int tc2_finish(const Tc2Work& w, int nocc, int queued, double* P, hipStream_t st
     hipLaunchKernelGGL(tc2_finish_kernel, dim3(nblk((long long)m * m)), dim3(256), 0, st,
                        (const double*)w.pair[0], (const double*)w.pair[1], (const double*)w.t, (const int*)w.stop,
                        queued, (double)nocc, m, w.ld, P);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// --------------------------------------------------------------------------
+// Scalars of one SCF cycle in two launches: dot products of up to
+// SCF_MAXDOT (x, y) pairs and the max / max-abs of up to two arrays, all over
+// n elements.  Pass 1: every block reduces a grid-strided share into
+// partial[block][out]; pass 2 (one block) reduces the partials.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double scf_block_reduce(double v, double* red, bool is_max)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = is_max ? fmax(v, w) : v + w;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double r = red[0];
+    for (int k = 1; k < nw; ++k) r = is_max ? fmax(r, red[k]) : r + red[k];
+    return r;
+}
+
+__global__ void __launch_bounds__(256) scf_dots_kernel(const ScfDotArgs a, double* partial)
+{
+    __shared__ double red[4];
+    const int nout = a.npair + 2 * a.nr;
+    for (int o = 0; o < nout; ++o) {
+        double v;
+        const bool is_max = o >= a.npair;
+        if (!is_max) {
+            v = 0.0;
+            const double* __restrict__ x = a.x[o];
+            const double* __restrict__ y = a.y[o];
+            for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < a.n;
+                 e += (long long)gridDim.x * blockDim.x)
+                v += x[e] * y[e];
+        } else {
+            const int j = (o - a.npair) >> 1;
+            const bool absval = (o - a.npair) & 1;
+            const double* __restrict__ r = a.r[j];
+            v = -1e300;
+            for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < a.n;
+                 e += (long long)gridDim.x * blockDim.x)
+                v = fmax(v, absval ? fabs(r[e]) : r[e]);
+        }
+        v = scf_block_reduce(v, red, is_max);
+        if (threadIdx.x == 0) partial[(size_t)blockIdx.x * nout + o] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) scf_dots_final_kernel(const double* __restrict__ partial, int nblocks, int npair,
+                                                             int nout, double* out)
+{
+    __shared__ double red[4];
+    for (int o = 0; o < nout; ++o) {
+        const bool is_max = o >= npair;
+        double v = is_max ? -1e300 : 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+            const double w = partial[(size_t)b * nout + o];
+            v = is_max ? fmax(v, w) : v + w;
+        }
+        v = scf_block_reduce(v, red, is_max);
+        if (threadIdx.x == 0) out[o] = v;
+    }
+}
+
+int scf_dots(const ScfDotArgs& a, double* work, hipStream_t st)
+{
+    if (a.npair < 0 || a.npair > SCF_MAXDOT || a.nr < 0 || a.nr > 2 || a.n <= 0) return -1;
+    const int nout = a.npair + 2 * a.nr;
+    if (nout == 0) return 0;
+    const int nb = scf_dots_blocks(a.n);
+    double* partial = work;
+    double* out = work + (size_t)nb * nout;
+    hipLaunchKernelGGL(scf_dots_kernel, dim3(nb), dim3(256), 0, st, a, partial);
+    hipLaunchKernelGGL(scf_dots_final_kernel, dim3(1), dim3(256), 0, st, (const double*)partial, nb, a.npair, nout,
+                       out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// out = sum_i c_i m_i over n elements (the DIIS extrapolation).
+__global__ void __launch_bounds__(256) lincomb_kernel(const LinCombArgs a, double* out)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < a.n; e += (long long)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int i = 0; i < a.k; ++i) v += a.c[i] * a.m[i][e];
+        out[e] = v;
+    }
+}
+
+int lincomb(const LinCombArgs& a, double* out, hipStream_t st)
+{
+    if (a.k < 1 || a.k > SCF_MAXCOMB || a.n <= 0) return -1;
+    const long long nb = std::min<long long>(1024, (a.n + 255) / 256);
+    hipLaunchKernelGGL(lincomb_kernel, dim3((unsigned)nb), dim3(256), 0, st, a, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -31,6 +31,8 @@ class DIIS:
         A = S @ D @ F
         r = A - A.T
         t = X.T @ r @ X
+        if t.shape == r.shape:
+            return t
         out = r * 0.0
         n = t.shape[0]
         out[:n, :n] = t
@@ -110,6 +112,9 @@ class DIIS:
             coeffs = np.linalg.solve(self.matrix, rhs)[:-1]
         except np.linalg.LinAlgError:
             coeffs = np.linalg.lstsq(self.matrix, rhs, rcond=None)[0][:-1]
+        if self._is_tensor(F) and F.device.type == "cuda" and len(self.focks) <= 16:
+            from .engine import lincomb
+            return lincomb(coeffs, self.focks)
         out = F * 0.0
         for c, f in zip(coeffs, self.focks):
             out += float(c) * f

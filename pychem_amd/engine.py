@@ -184,6 +184,43 @@ class Engine:
         return out
 
 
+_SCF_WORK = {}
+
+
+def scf_dots(pairs, maxes=()):
+    """[sum x*y for (x, y) in pairs] + [max r, max |r| for r in maxes] for
+    equally sized contiguous float64 device tensors, in two library launches
+    and one copy back (pq_scf_dots)."""
+    L = _lib.lib()
+    tensors = [t for p in pairs for t in p] + list(maxes)
+    n = tensors[0].numel()
+    for t in tensors:
+        if t.numel() != n or not t.is_contiguous() or t.dtype != torch.float64:
+            raise ValueError("scf_dots: equally sized contiguous float64 tensors")
+    nout = len(pairs) + 2 * len(maxes)
+    key = (n, nout, tensors[0].device)
+    if key not in _SCF_WORK:
+        _SCF_WORK[key] = torch.empty(int(L.pq_scf_dots_work(n, nout)), dtype=torch.float64,
+                                     device=tensors[0].device)
+    VP = ctypes.c_void_p
+    xs = (VP * max(1, len(pairs)))(*[x.data_ptr() for x, _ in pairs])
+    ys = (VP * max(1, len(pairs)))(*[y.data_ptr() for _, y in pairs])
+    rs = (VP * 2)(*([r.data_ptr() for r in maxes] + [None] * (2 - len(maxes))))
+    out = (ctypes.c_double * nout)()
+    check(L.pq_scf_dots(xs, ys, len(pairs), rs, len(maxes), n, _ptr(_SCF_WORK[key]), out, _stream()), "pq_scf_dots")
+    return list(out)
+
+
+def lincomb(coeffs, mats):
+    """sum_i c_i m_i for contiguous float64 device tensors (pq_lincomb)."""
+    out = torch.empty_like(mats[0])
+    VP = ctypes.c_void_p
+    ms = (VP * len(mats))(*[m.contiguous().data_ptr() for m in mats])
+    cs = (ctypes.c_double * len(mats))(*[float(c) for c in coeffs])
+    check(_lib.lib().pq_lincomb(ms, cs, len(mats), mats[0].numel(), _ptr(out), _stream()), "pq_lincomb")
+    return out
+
+
 def release_cache():
     """Return the library's cached device memory to the driver (bases keep
     released buffers cached for the next basis)."""

@@ -508,6 +508,19 @@ def _result(molecule, state, it, dE, conv, history, timings):
 DEVICE_ENERGY = "variational"
 
 
+def _scf_scalars(pairs, maxes):
+    """Dot products of the pairs, then max and max-abs of each array in
+    ``maxes``: the library's one-call reduction for device tensors, torch
+    for host tensors."""
+    if pairs[0][0].device.type == "cuda":
+        from .engine import scf_dots
+        return scf_dots(pairs, maxes)
+    out = [float((x * y).sum()) for x, y in pairs]
+    for r in maxes:
+        out += [float(r.max()), float(r.abs().max())]
+    return out
+
+
 def _scf_device(molecule, state, conv, maxiter, use_diis, diis_start, verbose, timing, t0, timings):
     """The SCF iteration on device tensors: J/K, Fock, energy test (see
     DEVICE_ENERGY), DIIS extrapolation, new density from ``purify``.  With the
@@ -546,19 +559,27 @@ def _scf_device(molecule, state, conv, maxiter, use_diis, diis_start, verbose, t
             Fb = H + J + Kb
         ra = DIIS.residual(S, Da, Fa, X)
         rb = ra if restricted else DIIS.residual(S, Db, Fb, X)
-        grad = torch.maximum(ra.abs().max(), rb.abs().max())
         diis_now = use_diis and it > diis_start
         if DEVICE_ENERGY == "variational":
-            # energy, gradient and the DIIS scalars of both spins in one
-            # device-to-host transfer
-            parts = [0.5 * (((Da + Db) * H).sum() + (Da * Fa).sum() + (Db * Fb).sum()), grad]
-            na_st = 0
+            # energy terms, residual maxima and the DIIS overlaps of both
+            # spins in one library call (two launches, one copy back)
+            if restricted:
+                pairs = [(Da, H), (Da, Fa)]
+            else:
+                pairs = [(Da, H), (Db, H), (Da, Fa), (Db, Fb)]
+            ne = len(pairs)
+            da = db = []
             if diis_now:
-                sa = state.AlphaDIIS.stats(ra)
-                na_st = len(sa)
-                parts += sa + ([] if restricted else state.BetaDIIS.stats(rb))
-            vals = torch.stack(parts).tolist()
-            Ev, gmax = vals[0], vals[1]
+                da = [(ra, r) for r in state.AlphaDIIS.residuals] + [(ra, ra)]
+                db = [] if restricted else [(rb, r) for r in state.BetaDIIS.residuals] + [(rb, rb)]
+            maxes = [ra] if restricted else [ra, rb]
+            vals = _scf_scalars(pairs + da + db, maxes)
+            e = vals[:ne]
+            Ev = e[0] + e[1] if restricted else 0.5 * sum(e)
+            mx = vals[ne + len(da) + len(db):]
+            gmax = max(mx[1], mx[3]) if len(mx) == 4 else mx[1]
+            pre_a = mx[0:2] + vals[ne:ne + len(da)]
+            pre_b = None if restricted else mx[2:4] + vals[ne + len(da):ne + len(da) + len(db)]
             dE = 1.0 if it == 1 else Ev - E
             E = Ev
             if abs(dE) <= conv and gmax > conv ** 0.5:
@@ -569,11 +590,13 @@ def _scf_device(molecule, state, conv, maxiter, use_diis, diis_start, verbose, t
             if abs(dE) <= conv:
                 break
             if diis_now:
-                Fa = state.AlphaDIIS.extrapolate(ra, Fa, pre=vals[2:2 + na_st])
-                Fb = Fa if restricted else state.BetaDIIS.extrapolate(rb, Fb, pre=vals[2 + na_st:])
-        elif diis_now:
-            Fa = state.AlphaDIIS.extrapolate(ra, Fa)
-            Fb = Fa if restricted else state.BetaDIIS.extrapolate(rb, Fb)
+                Fa = state.AlphaDIIS.extrapolate(ra, Fa, pre=pre_a)
+                Fb = Fa if restricted else state.BetaDIIS.extrapolate(rb, Fb, pre=pre_b)
+        else:
+            grad = torch.maximum(ra.abs().max(), rb.abs().max())
+            if diis_now:
+                Fa = state.AlphaDIIS.extrapolate(ra, Fa)
+                Fb = Fa if restricted else state.BetaDIIS.extrapolate(rb, Fb)
         Da = X @ purify(Xt @ Fa @ X, na) @ Xt
         Db = Da if restricted else X @ purify(Xt @ Fb @ X, nb) @ Xt
         if DEVICE_ENERGY == "variational":

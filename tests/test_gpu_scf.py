@@ -108,3 +108,23 @@ def test_pq_purify_matches_eigenvector_projector(gpu, m, nocc):
     ref = Q[:, :nocc] @ Q[:, :nocc].T
     assert np.abs(P - ref).max() < 1e-9
     assert abs(np.trace(P) - nocc) < 1e-9
+
+
+@pytest.mark.parametrize("n", [7, 194, 1000])
+def test_scf_dots_and_lincomb_match_torch(gpu, n):
+    """pq_scf_dots (dot products, max, max-abs) and pq_lincomb against torch
+    on random n x n matrices, up to the 24-pair and 16-matrix limits."""
+    import torch
+    from pychem_amd.engine import lincomb, scf_dots
+    g = torch.Generator(device="cpu").manual_seed(n)
+    M = [torch.randn(n, n, generator=g, dtype=torch.float64).cuda() for _ in range(17)]
+    pairs = [(M[i % 17], M[(3 * i + 1) % 17]) for i in range(24)]
+    out = scf_dots(pairs, [M[0], M[5]])
+    ref = [float((x * y).sum()) for x, y in pairs] + [float(M[0].max()), float(M[0].abs().max()),
+                                                       float(M[5].max()), float(M[5].abs().max())]
+    assert np.allclose(out, ref, rtol=1e-12, atol=1e-10 * n)
+    assert out[-4:] == ref[-4:]                      # maxima are exact
+    c = np.linspace(-1.0, 2.0, 16)
+    L = lincomb(c, M[:16])
+    R = sum(float(ci) * m for ci, m in zip(c, M[:16]))
+    assert torch.allclose(L, R, rtol=1e-13, atol=1e-12)
