@@ -40,6 +40,17 @@ struct EriArgs {
     int nq;
 };
 
+// Several classes in one launch: blocks [block0[j], block0[j + 1]) run
+// class slot which[j] (eri_fused_k) on quartets cls[j].
+#define ERI_MAXFUSED 8
+template <typename real>
+struct EriFusedArgs {
+    EriArgs<real> cls[ERI_MAXFUSED];
+    int which[ERI_MAXFUSED];
+    int block0[ERI_MAXFUSED + 1];
+    int n;
+};
+
 struct OneeArgs {
     const int* pair_list;
     const PairHdr* pairs;

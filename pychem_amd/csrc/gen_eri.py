@@ -174,9 +174,8 @@ def gen_eri_class(la, lb, lc, ld):
 
     lines = []
     lines.append("template <typename real>")
-    lines.append("__global__ void __launch_bounds__(ERI_BLOCK) eri_k%s(const EriArgs<real> args)" % tag)
+    lines.append("__device__ __forceinline__ void eri_body_%s(const EriArgs<real>& args, const int q)" % tag)
     lines.append("{")
-    lines.append("  const int q = blockIdx.x * blockDim.x + threadIdx.x;")
     lines.append("  const int nq = args.nq;")
     lines.append("  if (q >= nq) return;")
     lines.append("  const int2 pr = args.quartets[q];")
@@ -365,6 +364,11 @@ def jk_tile(la, lb, lc, ld, budget=72):
     return ta, tb, (n1 // ta) * (n2 // tb)
 
 
+# classes whose one-thread-per-quartet kernels spill (scratch) with this
+# generator's code; they share one fused launch in the ERI pass
+FUSED = ["2111", "2121", "2211", "2220", "2221", "2222"]
+
+
 def main():
     here = os.path.dirname(os.path.abspath(__file__))
     gdir = os.path.join(here, "gen")
@@ -378,7 +382,13 @@ def main():
         code, ncomp, nbody = gen_eri_class(la, lb, lc, ld)
         table.append((la, lb, lc, ld, ncomp, X, Y, nbody))
         flop_table[tag] = dict(gen_eri_class.flops, jk_tile=list(jk_tile(la, lb, lc, ld)))
-        src = ["// generated by gen_eri.py -- do not edit", '#include "../eri_common.h"', code]
+        _write_if_changed(os.path.join(gdir, "body_%s.h" % tag),
+                          "\n".join(["// generated by gen_eri.py -- do not edit", "#pragma once",
+                                     '#include "../eri_common.h"', code]) + "\n")
+        src = ["// generated by gen_eri.py -- do not edit", '#include "body_%s.h"' % tag,
+               "template <typename real>",
+               "__global__ void __launch_bounds__(ERI_BLOCK) eri_k%s(const EriArgs<real> args)" % tag,
+               "{ eri_body_%s<real>(args, blockIdx.x * blockDim.x + threadIdx.x); }" % tag]
         jsrc = ["// generated by gen_eri.py -- do not edit", '#include "../jk_common.h"']
         for rn, rt in (("d", "double"), ("f", "float")):
             src.append("void eri_run_%s_%s(const EriArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
@@ -397,6 +407,28 @@ def main():
         jsrc.append("{ hipLaunchKernelGGL((jk_tile_kernel<%d, %d, %d, %d, %d, %d>), dim3(a.grid, %d), dim3(block), "
                     "sizeof(double) * %d * a.nmax, st, a); }" % (la, lb, lc, ld, ta, tb, nt, 2 * (ta + tb)))
         _write_if_changed(os.path.join(gdir, "jk_%s.hip" % tag), "\n".join(jsrc) + "\n")
+    # the heavy classes (their one-thread-per-quartet kernels use scratch)
+    # in one launch: the block index selects the class, so they run side by
+    # side on one queue instead of one after another
+    fz = ["// generated by gen_eri.py -- do not edit", '#include "../eri_common.h"']
+    fz += ['#include "body_%s.h"' % t for t in FUSED]
+    fz.append("template <typename real>")
+    fz.append("__global__ void __launch_bounds__(ERI_BLOCK) eri_fused_k(const EriFusedArgs<real> a)")
+    fz.append("{")
+    fz.append("  const int b = blockIdx.x;")
+    fz.append("  int j = 0;")
+    fz.append("  while (j + 1 < a.n && b >= a.block0[j + 1]) ++j;")
+    fz.append("  const int q = (b - a.block0[j]) * ERI_BLOCK + threadIdx.x;")
+    fz.append("  switch (a.which[j]) {")
+    for k, t in enumerate(FUSED):
+        fz.append("  case %d: eri_body_%s<real>(a.cls[j], q); break;" % (k, t))
+    fz.append("  default: break;")
+    fz.append("  }")
+    fz.append("}")
+    for rn, rt in (("d", "double"), ("f", "float")):
+        fz.append("void eri_fused_run_%s(const EriFusedArgs<%s>& a, hipStream_t st)" % (rn, rt))
+        fz.append("{ if (a.n > 0) hipLaunchKernelGGL(eri_fused_k<%s>, dim3(a.block0[a.n]), dim3(ERI_BLOCK), 0, st, a); }" % rt)
+    _write_if_changed(os.path.join(gdir, "eri_fused.hip"), "\n".join(fz) + "\n")
     onee = ["// generated by gen_eri.py -- do not edit", '#include "../eri_common.h"']
     for la, lb in PAIRS:
         onee.append(gen_onee(la, lb))
@@ -421,6 +453,14 @@ def main():
             d.append("void jk_run_%s_%s(const JkArgs<%s>&, hipStream_t);" % (tag, rn, rt))
             d.append("void scatter_run_%s_%s(const ScatterArgs<%s>&, hipStream_t);" % (tag, rn, rt))
     d.append("void onee_run(int pcls, const OneeArgs& a, hipStream_t st);")
+    d.append("// classes of the fused heavy-class launch: slot in eri_fused_k, or -1")
+    d.append("#define ERI_NFUSED %d" % len(FUSED))
+    d.append("static const int eri_fused_slot[ERI_NCLASS] = {")
+    d.append("  " + ", ".join(str(FUSED.index("%d%d%d%d" % t[:4])) if "%d%d%d%d" % t[:4] in FUSED else "-1"
+                              for t in table))
+    d.append("};")
+    d.append("void eri_fused_run_d(const EriFusedArgs<double>&, hipStream_t);")
+    d.append("void eri_fused_run_f(const EriFusedArgs<float>&, hipStream_t);")
     d.append("// la, lb, lc, ld, components, bra pair class, ket pair class")
     d.append("static const int eri_class_table[ERI_NCLASS][7] = {")
     for la, lb, lc, ld, ncomp, X, Y, nbody in table:

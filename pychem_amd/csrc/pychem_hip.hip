@@ -1042,12 +1042,50 @@ extern "C" int pq_eri_compute(pq_basis* b, void* stream)
         HIPCHK(hipEventRecord(b->ev_fork, st));
         for (int k = 0; k < ns; ++k) HIPCHK(hipStreamWaitEvent(b->jk_side[k], b->ev_fork, 0));
     }
+    // the heavy classes (their kernels use scratch) in one launch per
+    // precision on the caller's stream, so that they run side by side while
+    // scratch stays on one queue; the rest below
+    for (int w = 0; w < 2; ++w) {
+        EriFusedArgs<double> fd = {};
+        EriFusedArgs<float> ff = {};
+        int nf = 0;
+        fd.block0[0] = ff.block0[0] = 0;
+        for (int c = 0; c < ERI_NCLASS; ++c) {
+            EriClassStore& s = w ? b->store32[c] : b->store64[c];
+            if (eri_fused_slot[c] < 0 || !s.nq) continue;
+            EriArgs<double> ad{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p,
+                               (const PrimPair<double>*)b->d_prims.p, (double*)s.vals.p, s.nq};
+            EriArgs<float> af{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p,
+                              (const PrimPair<double>*)b->d_prims.p, (float*)s.vals.p, s.nq};
+            fd.cls[nf] = ad;
+            ff.cls[nf] = af;
+            fd.which[nf] = ff.which[nf] = eri_fused_slot[c];
+            fd.block0[nf + 1] = ff.block0[nf + 1] = fd.block0[nf] + (s.nq + ERI_BLOCK - 1) / ERI_BLOCK;
+            ++nf;
+        }
+        fd.n = ff.n = nf;
+        if (!nf) continue;
+        if (b->timing)
+            for (int c = 0; c < ERI_NCLASS; ++c)
+                if (eri_fused_slot[c] >= 0 && (w ? b->store32[c] : b->store64[c]).nq)
+                    HIPCHK(hipEventRecord(b->ev_cls[2 * c + w][0], st));
+        if (w) eri_fused_run_f(ff, st);
+        else eri_fused_run_d(fd, st);
+        HIPCHK(hipGetLastError());
+        if (b->timing)
+            for (int c = 0; c < ERI_NCLASS; ++c)
+                if (eri_fused_slot[c] >= 0 && (w ? b->store32[c] : b->store64[c]).nq) {
+                    HIPCHK(hipEventRecord(b->ev_cls[2 * c + w][1], st));
+                    b->ev_cls_used[2 * c + w] = 1;
+                }
+    }
     double load[4] = {0.0, 0.0, 0.0, 0.0};
     for (const auto& o : order) {
         int k = 0;
         for (int t = 1; t < ns; ++t)
             if (load[t] < load[k]) k = t;
         const int c = o.second / 2, w = o.second % 2, slot = o.second;
+        if (eri_fused_slot[c] >= 0) continue;
         const bool spills = (w ? eri_scratch_table_f[c] : eri_scratch_table_d[c])() > 0;
         if (!spills) load[k] += o.first;
         hipStream_t sk = (ns > 1 && !spills) ? b->jk_side[k] : st;
