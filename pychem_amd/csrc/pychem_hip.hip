@@ -735,6 +735,27 @@ extern "C" int pq_schwarz(pq_basis* b, double* pair_bound_host, void* stream)
         HIPCHK(hipMemcpyAsync(dq.p, hq.data(), sizeof(int2) * nq_all, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(dids.p, hids.data(), sizeof(int) * nq_all, hipMemcpyHostToDevice, st));
     }
+    // the light diagonal classes on the side streams, the heavy ones (their
+    // kernels use scratch) in one fused launch on the caller's stream; each
+    // class's bounds are extracted on the stream that computed it
+    const int ns = std::min(4, std::max(1, b->jk_nstreams));
+    if (ns > 1) {
+        if (ensure_jk_streams(b)) return -1;
+        HIPCHK(hipEventRecord(b->ev_fork, st));
+        for (int k = 0; k < ns; ++k) HIPCHK(hipStreamWaitEvent(b->jk_side[k], b->ev_fork, 0));
+    }
+    auto extract = [&](int cls, int n, size_t oq, size_t ov, hipStream_t sk) {
+        const int l1 = eri_class_table[cls][0], l2 = eri_class_table[cls][1];
+        hipLaunchKernelGGL(schwarz_extract_kernel, dim3((n + 127) / 128), dim3(128), 0, sk,
+                           (const double*)((double*)dv.p + ov), (const int*)dids.p + oq, n, ncart_l(l1), ncart_l(l2),
+                           (const PairHdr*)b->d_pairs.p, b->any_spher ? (const int*)b->d_spher.p : (const int*)nullptr,
+                           (double*)b->d_bound.p);
+    };
+    EriFusedArgs<double> fd = {};
+    int nf = 0, side = 0;
+    int hcls[ERI_MAXFUSED], hn[ERI_MAXFUSED];
+    size_t hoq[ERI_MAXFUSED], hov[ERI_MAXFUSED];
+    fd.block0[0] = 0;
     size_t oq = 0, ov = 0;
     for (int X = 0; X < 6; ++X) {
         const int n = (int)b->class_pairs[X].size();
@@ -743,15 +764,35 @@ extern "C" int pq_schwarz(pq_basis* b, double* pair_bound_host, void* stream)
         const int ncomp = eri_class_table[cls][4];
         const int2* q = (const int2*)dq.p + oq;
         double* v = (double*)dv.p + ov;
-        if (run_class<double>(b, cls, q, n, v, st)) return -1;
-        const int l1 = eri_class_table[cls][0], l2 = eri_class_table[cls][1];
-        hipLaunchKernelGGL(schwarz_extract_kernel, dim3((n + 127) / 128), dim3(128), 0, st, (const double*)v,
-                           (const int*)dids.p + oq, n, ncart_l(l1), ncart_l(l2), (const PairHdr*)b->d_pairs.p,
-                           b->any_spher ? (const int*)b->d_spher.p : (const int*)nullptr, (double*)b->d_bound.p);
+        if (eri_fused_slot[cls] >= 0 && nf < ERI_MAXFUSED) {
+            fd.cls[nf] = EriArgs<double>{q, (const PairHdr*)b->d_pairs.p, (const PrimPair<double>*)b->d_prims.p, v, n};
+            fd.which[nf] = eri_fused_slot[cls];
+            fd.block0[nf + 1] = fd.block0[nf] + (n + ERI_BLOCK - 1) / ERI_BLOCK;
+            hcls[nf] = cls;
+            hn[nf] = n;
+            hoq[nf] = oq;
+            hov[nf] = ov;
+            ++nf;
+        } else {
+            hipStream_t sk = (ns > 1 && (eri_scratch_table_d[cls])() == 0) ? b->jk_side[side++ % ns] : st;
+            if (run_class<double>(b, cls, q, n, v, sk)) return -1;
+            extract(cls, n, oq, ov, sk);
+        }
         HIPCHK(hipGetLastError());
         oq += n;
         ov += (size_t)n * ncomp;
     }
+    if (nf) {
+        fd.n = nf;
+        eri_fused_run_d(fd, st);
+        for (int j = 0; j < nf; ++j) extract(hcls[j], hn[j], hoq[j], hov[j], st);
+        HIPCHK(hipGetLastError());
+    }
+    if (ns > 1)
+        for (int k = 0; k < ns; ++k) {
+            HIPCHK(hipEventRecord(b->ev_join[k], b->jk_side[k]));
+            HIPCHK(hipStreamWaitEvent(st, b->ev_join[k], 0));
+        }
     HIPCHK(hipStreamSynchronize(st));   // hq, hids and the scratch go out of scope
     dq.release(); dv.release(); dids.release();
     b->bound.resize(b->npair);
