@@ -263,6 +263,104 @@ __device__ __forceinline__ void jk_tile_quartets(const JkTileList<real>& L, int 
             }
 }
 
+// Classes with at least three ket components (every ket but ss)
+// digest one (quartet, ket row c) unit per thread instead of a quartet: N3
+// times the independent units of an item, and only one row's values in
+// registers.  Unit u of the range goes to row u / len, quartet q0 + u % len
+// (consecutive lanes read consecutive quartets of one row), and to thread
+// (u + shift) mod blockDim.  The K contributions of row c go to the LDS rows
+// at once, the ket J values to their own jkbuf slots as in the quartet loop.
+template <int LC, int LD>
+struct JkRowSplit {
+    static constexpr bool value = (LC + 1) * (LC + 2) / 2 * ((LD + 1) * (LD + 2) / 2) >= 3;
+};
+
+template <int LA, int LB, int LC, int LD, int TA, int TB, typename real>
+__device__ __forceinline__ void jk_tile_rows(const JkTileList<real>& L, int q0, int q1, int shift,
+                                             const double* drows, double* krows, int n,
+                                             const double* __restrict__ Dj, double sbra,
+                                             const double (&d12)[TA][TB], double (&j12)[TA][TB], int a0, int b0,
+                                             int tile)
+{
+    constexpr int N2 = (LB + 1) * (LB + 2) / 2;
+    constexpr int N3 = (LC + 1) * (LC + 2) / 2, N4 = (LD + 1) * (LD + 2) / 2;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int len = q1 - q0;
+    if (len <= 0) return;
+    const int total = N3 * len;
+    const size_t nq = L.nq;
+    const real* __restrict__ vals = L.vals;
+    double* __restrict__ jkbuf = L.jkbuf;
+    const int first = (tid - shift % nt + nt) % nt;
+    for (int u = first; u < total; u += nt) {
+        const int c = u / len;
+        const int q = q0 + (u - c * len);
+        const unsigned w = L.kets[q];
+        const int o3 = w & 0x7fff, o4 = (w >> 15) & 0x7fff;
+        double sc = sbra;
+        if ((w >> 30) & 1u) sc *= 0.5;
+        if (w >> 31) sc *= 0.5;
+        real vv[N4][TA][TB];
+        double dj[N4];
+#pragma unroll
+        for (int d = 0; d < N4; ++d) {
+            dj[d] = Dj[(size_t)(o3 + c) * n + o4 + d];
+#pragma unroll
+            for (int x = 0; x < TA; ++x)
+#pragma unroll
+                for (int y = 0; y < TB; ++y)
+                    vv[d][x][y] = vals[(size_t)((((a0 + x) * N2 + b0 + y) * N3 + c) * N4 + d) * nq + q];
+        }
+        double d13[TA], d23[TB], k13[TA], k23[TB];
+#pragma unroll
+        for (int x = 0; x < TA; ++x) {
+            d13[x] = drows[x * n + o3 + c];
+            k13[x] = 0.0;
+        }
+#pragma unroll
+        for (int y = 0; y < TB; ++y) {
+            d23[y] = drows[(TA + y) * n + o3 + c];
+            k23[y] = 0.0;
+        }
+#pragma unroll
+        for (int d = 0; d < N4; ++d) {
+            double d14[TA], d24[TB], k14[TA], k24[TB];
+#pragma unroll
+            for (int x = 0; x < TA; ++x) {
+                d14[x] = drows[x * n + o4 + d];
+                k14[x] = 0.0;
+            }
+#pragma unroll
+            for (int y = 0; y < TB; ++y) {
+                d24[y] = drows[(TA + y) * n + o4 + d];
+                k24[y] = 0.0;
+            }
+            double jcd = 0.0;
+#pragma unroll
+            for (int x = 0; x < TA; ++x)
+#pragma unroll
+                for (int y = 0; y < TB; ++y) {
+                    const double v = sc * (double)vv[d][x][y];
+                    j12[x][y] += dj[d] * v;
+                    jcd += d12[x][y] * v;
+                    k13[x] += d24[y] * v;
+                    k14[x] += d23[y] * v;
+                    k23[y] += d14[x] * v;
+                    k24[y] += d13[x] * v;
+                }
+            jkbuf[(size_t)(tile * N3 * N4 + c * N4 + d) * nq + q] = 4.0 * jcd;
+#pragma unroll
+            for (int x = 0; x < TA; ++x) atomicAdd(&krows[x * n + o4 + d], 2.0 * k14[x]);
+#pragma unroll
+            for (int y = 0; y < TB; ++y) atomicAdd(&krows[(TA + y) * n + o4 + d], 2.0 * k24[y]);
+        }
+#pragma unroll
+        for (int x = 0; x < TA; ++x) atomicAdd(&krows[x * n + o3 + c], 2.0 * k13[x]);
+#pragma unroll
+        for (int y = 0; y < TB; ++y) atomicAdd(&krows[(TA + y) * n + o3 + c], 2.0 * k23[y]);
+    }
+}
+
 template <int LA, int LB, int LC, int LD, int TA, int TB>
 __device__ __forceinline__ void jk_tile_item(const JkTileArgs& a, int item, double* lds, int tile)
 {
@@ -316,12 +414,23 @@ __device__ __forceinline__ void jk_tile_item(const JkTileArgs& a, int item, doub
             j12[x][y] = 0.0;
         }
     __syncthreads();
-    jk_tile_quartets<LA, LB, LC, LD, TA, TB, double>(a.l64, it.start, it.end, 0, drows, krows, n, Dj, sbra, d12,
-                                                     j12, a0, b0, tile);
-    if (a.items32) {
-        const int2 r32 = a.items32[item];
-        jk_tile_quartets<LA, LB, LC, LD, TA, TB, float>(a.l32, r32.x, r32.y, it.end - it.start, drows, krows, n, Dj,
-                                                        sbra, d12, j12, a0, b0, tile);
+    if (JkRowSplit<LC, LD>::value) {
+        constexpr int N3 = (LC + 1) * (LC + 2) / 2;
+        jk_tile_rows<LA, LB, LC, LD, TA, TB, double>(a.l64, it.start, it.end, 0, drows, krows, n, Dj, sbra, d12, j12,
+                                                     a0, b0, tile);
+        if (a.items32) {
+            const int2 r32 = a.items32[item];
+            jk_tile_rows<LA, LB, LC, LD, TA, TB, float>(a.l32, r32.x, r32.y, N3 * (it.end - it.start), drows, krows,
+                                                        n, Dj, sbra, d12, j12, a0, b0, tile);
+        }
+    } else {
+        jk_tile_quartets<LA, LB, LC, LD, TA, TB, double>(a.l64, it.start, it.end, 0, drows, krows, n, Dj, sbra, d12,
+                                                         j12, a0, b0, tile);
+        if (a.items32) {
+            const int2 r32 = a.items32[item];
+            jk_tile_quartets<LA, LB, LC, LD, TA, TB, float>(a.l32, r32.x, r32.y, it.end - it.start, drows, krows, n,
+                                                            Dj, sbra, d12, j12, a0, b0, tile);
+        }
     }
 #pragma unroll
     for (int x = 0; x < TA; ++x)

@@ -1269,7 +1269,11 @@ static int jk_launch_class(pq_basis* b, int c, const double* Dj, const double* D
             a.nitems = items;
             const int tuned = s64.jk_grid > 0 ? s64.jk_grid : s32.jk_grid;
             a.grid = (tuned > 0 && tuned < items) ? tuned : items;
-            const double mean = (double)((u64 ? s64.nq : 0) + (u32 ? s32.nq : 0)) / std::max(1, items);
+            // threads per block from the mean work units per item: quartets,
+            // or (quartet, ket row) pairs where the kernel splits the rows
+            const int n3 = ncart_l(eri_class_table[c][2]), n4 = ncart_l(eri_class_table[c][3]);
+            const double units = (n3 * n4 >= 3) ? (double)n3 : 1.0;
+            const double mean = units * (double)((u64 ? s64.nq : 0) + (u32 ? s32.nq : 0)) / std::max(1, items);
             const int block = mean > 160.0 ? 256 : (mean > 80.0 ? 128 : 64);
             if (items > 0) jktile_launch(c, a, block, st);
         }
