@@ -11,8 +11,8 @@ pychem_amd/data/sad_atoms.npz), SCF with Pulay DIIS to |dE| < 1e-10.
 Prints ONE JSON line.  ``value`` = screened unique ERI shell quartets per
 second of whole-solve wall time (quartets computed per solve / seconds per
 solve).  ``roofline`` describes the dominant kernel of the solve: the J/K
-digestion of the costliest ERI class (one launch per SCF iteration), HBM
-bound, timed live with HIP events on the stream it runs on; its ``traffic``
+digestion of the ERI class with the most algorithmic bytes (one launch per
+SCF iteration), HBM bound, timed live with HIP events on the stream it runs on; its ``traffic``
 comes from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
 profiles/pmc_traffic.json when they were taken for that kernel.
 ``eri_roofline`` gives the ERI generation pass against the FP64 vector peak.
@@ -72,7 +72,7 @@ def jk_roofline(mol, reps=10):
     (pq_jk_class part 1) with HIP events on the current stream and price its
     algorithmic HBM bytes per launch: stored ERI values nq*ncomp*8, packed ket
     records nq*4, work items nitems*16, ket J values written nq*ntiles*N3*N4*8.
-    Returns the per-class table and the dominant (slowest) class."""
+    Returns the per-class table and the dominant class (most bytes)."""
     import ctypes
     import torch
     from pychem_amd import _lib
@@ -107,7 +107,9 @@ def jk_roofline(mol, reps=10):
         nbytes = nq * s["ncomp"] * 8 + nq * 4 + nit.value * 16 + nq * nt * ncd * 8
         out.append(dict(cls=s["cls"], kernel="jk_tile_kernel<%d, %d, %d, %d, %d, %d>" % (la, lb, lc, ld, ta, tb),
                         nq=nq, items=nit.value, bytes=nbytes, ms=ms, gbs=nbytes / (ms * 1e-3) / 1e9))
-    dom = max(out, key=lambda c: c["ms"])
+    # the class with the most algorithmic bytes: the J/K launch that moves
+    # the most data (several classes are within noise of each other in time)
+    dom = max(out, key=lambda c: c["bytes"])
     return out, dom
 
 
