@@ -210,6 +210,36 @@ int pq_scf_dots(const double* const* x_dev, const double* const* y_dev, int npai
  * Methods/hf_extensions/diis.py:108-118 in one launch). */
 int pq_lincomb(const double* const* m_dev, const double* c_host, int k, long long n, double* out_dev, void* stream);
 
+/* The whole SCF iteration in one call: the loop of Methods/hartree_fock.py:
+ * 79-140 (J/K from pq_jk, Fock matrices, Pulay DIIS as in
+ * Methods/hf_extensions/diis.py:14-155, new densities) with the densities
+ * from pq_purify and the convergence test on the variational energy
+ * E[D] = sum D (H + F[D]) / 2 of each cycle's input density: stop when
+ * |E - E_prev| <= conv and max |X^T (S D F - F D S) X| <= sqrt(conv).
+ * n = final basis size, X = n x nx orthogonaliser (X^T S X = 1); T = the
+ * n x ncart Cartesian-to-final transform, or NULL when ncart == n.  H, S, X,
+ * T, Da, Db, Fa, Fb are device matrices (row-major).  Da (and Db when
+ * unrestricted) hold the guess on entry and the last input density on
+ * exit (after maxiter cycles: the last new density); Fa (Fb) its own Fock
+ * matrix (after maxiter cycles: the last extrapolated one).  Restricted:
+ * nalpha == nbeta, J from 2 Da, Db / Fb may be NULL or receive copies.
+ * DIIS: a residual enters while |max r| < diis_threshold and max |r| >=
+ * 1e-14, from cycle diis_start + 1 when use_diis; the oldest vectors are
+ * dropped while more than diis_size (restricted <= 15, unrestricted <= 9)
+ * are kept or the bordered matrix's condition number exceeds
+ * diis_max_cond.  work: pq_scf_work(...) doubles, 256-byte aligned.
+ * energies (host, maxiter) receive each cycle's electronic energy, jk_ms
+ * (host, 2 maxiter, may be NULL; needs pq_set_timing) each J/K's time;
+ * result (host) = [E, dE, cycles (maxiter + 1 when not converged),
+ * purification steps].  One host copy of the cycle's scalars and the
+ * purification's own step read per cycle. */
+long long pq_scf_work(int n, int nx, int ncart, int unrestricted, int diis_size);
+int pq_scf_run(pq_basis* b, const double* H_dev, const double* S_dev, const double* X_dev, int n, int nx,
+               const double* T_dev, int ncart, int nalpha, int nbeta, int unrestricted, double conv, int maxiter,
+               int use_diis, int diis_start, double diis_threshold, double diis_max_cond, int diis_size,
+               double* Da_dev, double* Db_dev, double* Fa_dev, double* Fb_dev, double* work_dev, double* energies_host,
+               double* jk_ms_host, double* result_host, void* stream);
+
 /* MP2 energy sum over ovov [o1][v1][o2][v2] with D = e_i + e_j - e_a - e_b:
  * mode 0 closed shell sum v(2v - (ib|ja))/D; mode 1 same spin
  * 1/4 sum ((ia|jb) - (ib|ja))^2 / D; mode 2 opposite spin sum v^2 / D

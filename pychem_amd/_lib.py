@@ -26,6 +26,10 @@ SIGNATURES = {
                             ctypes.c_longlong, c_void_p, ctypes.POINTER(c_double), c_void_p]),
     "pq_lincomb": (c_int, [ctypes.POINTER(c_void_p), ctypes.POINTER(c_double), c_int, ctypes.c_longlong, c_void_p,
                            c_void_p]),
+    "pq_scf_work": (ctypes.c_longlong, [c_int, c_int, c_int, c_int, c_int]),
+    "pq_scf_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int,
+                           c_int, c_double, c_int, c_int, c_int, c_double, c_double, c_int, c_void_p, c_void_p,
+                           c_void_p, c_void_p, c_void_p, P_double, P_double, P_double, c_void_p]),
     "pq_cache_bytes": (ctypes.c_longlong, []),
     "pq_boys": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "pq_basis_create": (c_int, [c_int, P_int, P_int, P_int, P_double, P_double, P_double,

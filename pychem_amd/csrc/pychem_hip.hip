@@ -136,6 +136,8 @@ static int fail(const std::string& m)
     g_err = m;
     return -1;
 }
+// error path of the library's other translation units
+int pq_set_error(const char* m) { return fail(m); }
 #define HIPCHK(x)                                                                       \
     do {                                                                                \
         hipError_t e_ = (x);                                                            \

@@ -65,3 +65,6 @@ struct LinCombArgs {
     long long n;
 };
 int lincomb(const LinCombArgs& a, double* out, hipStream_t st);
+
+// Sets the calling thread's error message (pq_last_error) and returns -1.
+int pq_set_error(const char* m);

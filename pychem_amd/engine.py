@@ -131,6 +131,49 @@ class Engine:
         check(_lib.lib().pq_jk(self.h, _ptr(Dj), _ptr(Dk), _ptr(J), _ptr(K), _stream()), "pq_jk")
         return self.to_final(J), self.to_final(K)
 
+    def scf_run(self, H, S, X, Da, Db, nalpha, nbeta, unrestricted, conv, maxiter, use_diis=True, diis_start=1,
+                diis_threshold=float("inf"), diis_max_cond=1e14, diis_size=8, timing=False):
+        """The SCF iteration in one library call (pq_scf_run) from the guess
+        densities Da (Db), final basis, device float64.  Returns a dict with
+        the last input densities Da, Db and their Fock matrices Fa, Fb, the
+        electronic energy E, its last change dE, the cycle count, the
+        per-cycle electronic energies and the purification steps."""
+        if not self.eri_built:
+            raise _lib.HipError("build_eri() first")
+        L = _lib.lib()
+        n, nx = X.shape
+        ncart = self.ncart
+        unr = 1 if unrestricted else 0
+        key = (n, nx, ncart, unr, int(diis_size), self.device)
+        size = int(L.pq_scf_work(n, nx, ncart, unr, int(diis_size)))
+        if size < 0:
+            raise _lib.HipError("pq_scf_work: bad sizes")
+        if key not in _SCF_RUN_WORK:
+            # 256-byte alignment: over-allocate a line and start at the aligned offset
+            _SCF_RUN_WORK[key] = torch.empty(size + 32, dtype=torch.float64, device=self.device)
+        buf = _SCF_RUN_WORK[key]
+        work = buf[((-buf.data_ptr()) % 256) // 8:]
+        H, S, X = (t.contiguous() for t in (H, S, X))
+        Da = Da.clone().contiguous()
+        Db = Da if not unrestricted else Db.clone().contiguous()
+        Fa = torch.empty_like(Da)
+        Fb = Fa if not unrestricted else torch.empty_like(Da)
+        energies = np.zeros(max(1, maxiter))
+        jk = np.full(2 * max(1, maxiter), -1.0)
+        res = np.zeros(4)
+        T = self.T.contiguous() if not self.cartesian else None
+        check(L.pq_scf_run(self.h, _ptr(H), _ptr(S), _ptr(X), n, nx, None if T is None else _ptr(T), ncart,
+                           int(nalpha), int(nbeta), unr, float(conv), int(maxiter), 1 if use_diis else 0,
+                           int(diis_start), float(diis_threshold), float(diis_max_cond), int(diis_size),
+                           _ptr(Da), _ptr(Db) if unrestricted else None, _ptr(Fa), _ptr(Fb) if unrestricted else None,
+                           _ptr(work), dptr(energies), dptr(jk) if timing else None, dptr(res), _stream()),
+              "pq_scf_run")
+        it = int(res[2])
+        done = min(it, maxiter)
+        return dict(Da=Da, Db=Db, Fa=Fa, Fb=Fb, E=float(res[0]), dE=float(res[1]), iterations=it,
+                    purify_steps=int(res[3]), energies=list(energies[:done]),
+                    jk_ms=[float(x) for x in jk[:2 * done] if x >= 0.0])
+
     def eri_dense(self):
         """Dense screened (ab|cd) [nao]^4 in the final basis (device)."""
         n = self.ncart
@@ -185,6 +228,7 @@ class Engine:
 
 
 _SCF_WORK = {}
+_SCF_RUN_WORK = {}
 
 
 def scf_dots(pairs, maxes=()):

@@ -507,6 +507,12 @@ def _result(molecule, state, it, dE, conv, history, timings):
 # can pass through a small energy change far from convergence.
 DEVICE_ENERGY = "variational"
 
+# The device loop with the variational test runs inside the library
+# (Engine.scf_run, one call for the whole iteration) when the DIIS settings
+# fit it (a bounded subspace); NATIVE_LOOP = False keeps the same iteration
+# in Python, one library call per step.
+NATIVE_LOOP = True
+
 
 def _scf_scalars(pairs, maxes):
     """Dot products of the pairs, then max and max-abs of each array in
@@ -543,7 +549,24 @@ def _scf_device(molecule, state, conv, maxiter, use_diis, diis_start, verbose, t
     history = []
     t2 = time.perf_counter()
     Fa = Fb = None
-    while conv < abs(dE):
+    dopt = state.AlphaDIIS
+    native = (NATIVE_LOOP and hasattr(eng, "scf_run") and DEVICE_ENERGY == "variational" and maxiter >= 1
+              and dopt.max_size is not None
+              and 1 <= dopt.max_size <= (15 if restricted else 9))
+    if native:
+        r = eng.scf_run(H, S, X, Da, Db, na, nb, not restricted, conv, maxiter, use_diis=use_diis,
+                        diis_start=diis_start, diis_threshold=dopt.threshold, diis_max_cond=dopt.max_condition,
+                        diis_size=dopt.max_size, timing=getattr(eng, "timing_on", False))
+        Da, Db, Fa, Fb = r["Da"], r["Db"], r["Fa"], r["Fb"]
+        E, dE, it = r["E"], r["dE"], r["iterations"]
+        history = [e + molecule.NuclearRepulsion for e in r["energies"]]
+        if getattr(eng, "timing_on", False):
+            eng.jk_ms.extend(r["jk_ms"])
+        if verbose:
+            for k, e in enumerate(history):
+                print(" Cycle: %d  Total energy: %.12f  Change: %.3e" %
+                      (k + 1, e, 1.0 if k == 0 else e - history[k - 1]))
+    while not native and conv < abs(dE):
         it += 1
         if it > maxiter:
             break

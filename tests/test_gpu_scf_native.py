@@ -1,0 +1,109 @@
+"""The SCF iteration inside the library (pq_scf_run, Engine.scf_run) against
+the same iteration driven from Python one library call per step
+(hartree_fock.NATIVE_LOOP = False), and against the oracle SCF."""
+import numpy as np
+import pytest
+
+from pychem_amd import geometry, structures
+from oracle import scf as oscf
+
+pytestmark = pytest.mark.gpu
+
+H3 = [["H", 1.0, 0, 0, 0], ["H", 1.0, 1.0, 0, 0], ["H", 1.0, 0.4, 1.0, 0]]   # Tests/example1.inp geometry
+
+CASES = {
+    "water_sto3g_core": (lambda: structures.Molecule(geometry.water(), basis="STO-3G"), "CORE"),
+    "propane_631gs_6d_sad": (lambda: structures.Molecule(geometry.alkane(3, 0.02, 3010), basis="6-31G*",
+                                                         cartesian_l=[2]), "SAD"),
+    "methanol_631gs_5d_sad": (lambda: structures.Molecule(geometry.small_molecule("CH3OH", 0.05, 4),
+                                                          basis="6-31G*"), "SAD"),
+    "h3_doublet_uhf_core": (lambda: structures.Molecule(H3, basis="STO-3G", multiplicity=2), "CORE"),
+}
+
+
+def _both(monkeypatch, make, guess, **kw):
+    from pychem_amd import hartree_fock
+    m1 = make()
+    r1 = hartree_fock.do(m1, guess=guess, **kw)
+    with monkeypatch.context() as mp:
+        mp.setattr(hartree_fock, "NATIVE_LOOP", False)
+        m2 = make()
+        r2 = hartree_fock.do(m2, guess=guess, **kw)
+    return m1, r1, m2, r2
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_native_loop_matches_python_loop(gpu, monkeypatch, case):
+    make, guess = CASES[case]
+    m1, r1, m2, r2 = _both(monkeypatch, make, guess, conv=1e-10, maxiter=200)
+    assert r1["converged"] and r2["converged"]
+    assert abs(r1["energy"] - r2["energy"]) < 1e-9
+    # the first cycles see the same densities: energies agree to rounding
+    k = min(3, len(r1["history"]), len(r2["history"]))
+    assert np.allclose(r1["history"][:k], r2["history"][:k], rtol=0, atol=1e-9)
+    # the two loops round differently (the library forms the orbital gradient
+    # as P F' - F' P once D = X P X^T); a slowly converging SCF (the UHF
+    # doublet takes some 60 cycles) can then take a few cycles more or less
+    assert abs(r1["iterations"] - r2["iterations"]) <= max(1, r2["iterations"] // 6)
+    # both stop once max |orbital gradient| <= sqrt(conv) = 1e-5: densities
+    # agree to that order
+    assert np.allclose(r1["Da"], r2["Da"], rtol=0, atol=5e-5)
+    assert np.allclose(r1["Db"], r2["Db"], rtol=0, atol=5e-5)
+    assert len(r1["history"]) == r1["iterations"]
+
+
+@pytest.mark.parametrize("case", ["propane_631gs_6d_sad", "h3_doublet_uhf_core"])
+def test_native_loop_maxiter_exit_returns_extrapolated_state(gpu, monkeypatch, case):
+    """Stopped after 4 cycles: the new density and the extrapolated Fock
+    matrix of the last cycle, as the Python loop leaves them."""
+    make, guess = CASES[case]
+    m1, r1, m2, r2 = _both(monkeypatch, make, guess, conv=1e-12, maxiter=4)
+    assert not r1["converged"] and not r2["converged"]
+    assert r1["iterations"] == r2["iterations"] == 5
+    assert np.allclose(r1["history"], r2["history"], rtol=0, atol=1e-9)
+    assert np.allclose(r1["Da"], r2["Da"], rtol=0, atol=1e-9)
+    assert np.allclose(r1["Db"], r2["Db"], rtol=0, atol=1e-9)
+    Fa1 = m1.State.Alpha.Fock.numpy()
+    Fa2 = m2.State.Alpha.Fock.numpy()
+    assert np.allclose(Fa1, Fa2, rtol=0, atol=1e-9)
+    Fb1 = m1.State.Beta.Fock.numpy()
+    Fb2 = m2.State.Beta.Fock.numpy()
+    assert np.allclose(Fb1, Fb2, rtol=0, atol=1e-9)
+
+
+def test_native_loop_without_diis_and_late_diis_start(gpu, monkeypatch):
+    """use_diis=False and diis_start=3 follow the Python loop too."""
+    make, guess = CASES["water_sto3g_core"]
+    for kw in (dict(use_diis=False), dict(diis_start=3)):
+        m1, r1, m2, r2 = _both(monkeypatch, make, guess, conv=1e-10, maxiter=200, **kw)
+        assert r1["converged"] and r2["converged"]
+        assert abs(r1["energy"] - r2["energy"]) < 1e-9
+        assert abs(r1["iterations"] - r2["iterations"]) <= 1
+
+
+def test_native_loop_vs_oracle_uhf(gpu):
+    """UHF doublet H3 to 1e-10: the library's loop against the oracle SCF."""
+    from pychem_amd import hartree_fock
+    m = structures.Molecule(H3, basis="STO-3G", multiplicity=2)
+    r = hartree_fock.do(m, conv=1e-10, maxiter=200)
+    o = oscf.run_rhf(m, conv=1e-10, maxiter=200, diis=oscf.PULAY)
+    assert r["converged"]
+    assert abs(r["energy"] - o["energy"]) < 1e-8
+
+
+def test_scf_run_rejects_bad_arguments(gpu):
+    import torch
+    from pychem_amd import _lib, hartree_fock
+    m = structures.Molecule(geometry.water(), basis="STO-3G")
+    hartree_fock.do(m, conv=1e-8, maxiter=50)
+    eng = m.Engine
+    n = m.NOrbitals
+    dev = torch.device("cuda")
+    H = torch.zeros(n, n, dtype=torch.float64, device=dev)
+    X = torch.eye(n, dtype=torch.float64, device=dev)
+    with pytest.raises(_lib.HipError):
+        eng.scf_run(H, H, X, H, H, 5, 5, False, 1e-8, 10, diis_size=16)
+    with pytest.raises(_lib.HipError):
+        eng.scf_run(H, H, X, H, H, 5, 4, False, 1e-8, 10)        # restricted needs nalpha == nbeta
+    with pytest.raises(_lib.HipError):
+        eng.scf_run(H, H, X, H, H, 5, 5, True, 1e-8, 10, diis_size=10)
