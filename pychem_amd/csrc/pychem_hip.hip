@@ -1672,16 +1672,22 @@ extern "C" int pq_purify(const double* F, int m, int nocc, double* P, double* wo
     hipStream_t st = (hipStream_t)stream;
     Tc2Work w = tc2_work(work, m, max_steps);
     if (tc2_start(F, w, st)) return fail("pq_purify: launch failed");
-    // Queue the step count of this thread's previous call (plus a margin),
-    // then 16 at a time: the steps record where to stop and the ones queued
-    // past it do nothing, so one host round trip usually suffices.
+    // Queue the step count of this thread's previous call on the fast step
+    // kernel, 4 more on the early-exit one, then 16 at a time (early-exit):
+    // the steps record where to stop and the ones queued past it store
+    // nothing, so one host round trip usually suffices.
     static thread_local int usual = 44;
     std::vector<double> h(2 * (size_t)max_steps + 1);
     int queued = 0, stop = 0x7fffffff;
-    int chunk = std::min(max_steps, usual + 4);
-    for (;;) {
-        if (tc2_steps(w, nocc, tol, queued, chunk, st)) return fail("pq_purify: launch failed");
+    int chunk = std::min(max_steps, usual);
+    for (bool first = true;; first = false) {
+        if (tc2_steps(w, nocc, tol, queued, chunk, st, !first)) return fail("pq_purify: launch failed");
         queued += chunk;
+        if (first) {
+            const int extra = std::min(4, max_steps - queued);
+            if (extra > 0 && tc2_steps(w, nocc, tol, queued, extra, st, true)) return fail("pq_purify: launch failed");
+            queued += extra;
+        }
         HIPCHK(hipMemcpyAsync(h.data(), w.t, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         memcpy(&stop, &h[2 * (size_t)max_steps], sizeof stop);

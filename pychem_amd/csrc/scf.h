@@ -30,8 +30,9 @@ int tc2_start(const double* F, Tc2Work& w, hipStream_t st);
 // pair and traces of step k - 1 (X0 for k = 0) and writes X_k, X_k^2 and
 // their traces into slot k.  The step after the one whose input has
 // tr(X - X^2) < tol is the last; it records its index in the stop word and
-// later queued steps do nothing.
-int tc2_steps(const Tc2Work& w, int nocc, double tol, int k0, int steps, hipStream_t st);
+// later queued steps do nothing (early: such steps return at once; otherwise
+// they still load and multiply, and a needed step saves a memory round trip).
+int tc2_steps(const Tc2Work& w, int nocc, double tol, int k0, int steps, hipStream_t st, bool early = false);
 // P = X_n symmetrised, n - 1 the recorded last step (or queued - 1 when no
 // step recorded one), X_n from step n - 1's pair and traces.
 int tc2_finish(const Tc2Work& w, int nocc, int queued, double* P, hipStream_t st);

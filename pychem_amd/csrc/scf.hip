@@ -17,8 +17,9 @@
 // (the start symmetrises F, each step computes only the upper 32 x 32 output
 // tiles and mirrors them), so the right operand's columns are read as rows
 // of X too.  A block's eight waves each compute the whole 32 x 32 tile over a
-// share of k, straight from global memory with three 16-wide k blocks of
-// loads in flight at once, and one LDS pass sums the partial tiles.
+// share of k, straight from global memory with all of a wave's 16-wide k
+// blocks (up to four) of loads in flight at once, and one LDS pass sums the
+// partial tiles.
 // Products on v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and
 // B[l/16][l%16], result register r of lane l is D[4r + l/16][l%16].  Sizes here are a few hundred rows, so a step is
 // latency-bound: no barrier inside the k loop, one launch per step.
@@ -118,56 +119,23 @@ __device__ __forceinline__ double tc2_wave_sum(double v)
 }
 
 #define TC2_W 8   // waves per block (each sums a share of k)
-#define TC2_C 3   // k blocks of 16 whose loads a wave has in flight together
 
-// DIRECT: X_k = P (step 0); otherwise X_k = Q when up, else 2 P - Q.  All
-// of a round's loads (both matrices) are issued before `up` is used, so the
-// branch costs no memory round trip of its own.
-template <bool DIRECT>
-__device__ __forceinline__ void tc2_step_body(const Tc2Step& a, bool up, double (&red)[TC2_W][4][4][64])
-{
-    // upper tile (bi, bj), bi <= bj, from the linear block index
-    int t = blockIdx.x, bi = 0;
-    while (t >= a.nt - bi) {
-        t -= a.nt - bi;
-        ++bi;
-    }
-    const int bj = bi + t;
-    const int i0 = bi * TC2_T, j0 = bj * TC2_T;
-    const int ld = a.ld;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int li = lane & 15, lq = lane >> 4;
-    const double cp = up ? 0.0 : 2.0, cq = up ? 1.0 : -1.0;   // X_k = cp P + cq Q
-    // the block's own tile of X_k (and its mirror) is written at the end;
-    // its elements are loaded now, with the operands
-    constexpr int NX = TC2_T * TC2_T / (64 * TC2_W);
-    double xp[NX], xq[NX];
+// Operands of one round: the C k blocks kb = kb0 + c * TC2_W of 16 k each
+// (blocks past the end read the last block again; keep masks them to zero).
+// Inside a block, MFMA step s pairs lane group q with k = 16 kb + 4 q + s, so
+// a lane's four k values of one row are consecutive (one aligned 32-byte
+// load; the product is a sum over k in any order, the same order for both
+// operands).  Rows of the left operand: i0 + 16 u + li; columns of the right
+// operand (read as rows of the symmetric X): j0 + 16 v + li.
+template <bool DIRECT, int C>
+struct Tc2Round {
+    tc2_d4 pa[C][2], pb[C][2], qa[C][2], qb[C][2];
+    double keep[C];
+    __device__ __forceinline__ void load(const Tc2Step& a, int kb0, int nkb, int i0, int j0, int li, int lq)
+    {
+        const int ld = a.ld;
 #pragma unroll
-    for (int n = 0; n < NX; ++n) {
-        const int e = tid + 64 * TC2_W * n;
-        const size_t idx = (size_t)(i0 + (e >> 5)) * ld + j0 + (e & 31);
-        xp[n] = a.P[idx];
-        xq[n] = DIRECT ? 0.0 : a.Q[idx];
-    }
-    // Wave w sums the k blocks kb = w, w + TC2_W, ... of 16 k each, TC2_C of
-    // them per round with all their loads issued first.  Inside a block, MFMA
-    // step s pairs lane group q with k = 16 kb + 4 q + s, so a lane's four k
-    // values of one row are consecutive (one aligned 32-byte load; the
-    // product is a sum over k in any order, the same order for both
-    // operands).  Rows of the left operand: i0 + 16 u + li; columns of the
-    // right operand (read as rows of the symmetric X_k): j0 + 16 v + li.
-    // k blocks past the end read the last block again and are masked to zero.
-    const int nkb = ld / 16;
-    tc2_d4 acc[2][2];
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int v = 0; v < 2; ++v) acc[u][v] = (tc2_d4){0.0, 0.0, 0.0, 0.0};
-    for (int kb0 = wave; kb0 < nkb; kb0 += TC2_W * TC2_C) {
-        tc2_d4 pa[TC2_C][2], pb[TC2_C][2], qa[TC2_C][2], qb[TC2_C][2];
-        double keep[TC2_C];
-#pragma unroll
-        for (int c = 0; c < TC2_C; ++c) {
+        for (int c = 0; c < C; ++c) {
             const int kb = kb0 + c * TC2_W;
             const int k = 16 * min(kb, nkb - 1) + 4 * lq;
             keep[c] = (double)(kb < nkb);
@@ -182,8 +150,11 @@ __device__ __forceinline__ void tc2_step_body(const Tc2Step& a, bool up, double 
                 }
             }
         }
+    }
+    __device__ __forceinline__ void mul(double cp, double cq, tc2_d4 (&acc)[2][2]) const
+    {
 #pragma unroll
-        for (int c = 0; c < TC2_C; ++c) {
+        for (int c = 0; c < C; ++c) {
             tc2_d4 fa[2], fb[2];
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
@@ -199,6 +170,73 @@ __device__ __forceinline__ void tc2_step_body(const Tc2Step& a, bool up, double 
                         acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u][s], fb[v][s], acc[u][v], 0, 0, 0);
         }
     }
+};
+
+// Step k.  DIRECT (step 0): X_k = P.  Otherwise X_k = Q (X_{k-1}^2) when
+// tr X_{k-1}^2 is the closer to nocc, else 2 P - Q.  The first round's
+// operand loads (both matrices) are issued before the step's inputs (stop
+// word, traces) are read, so deciding the branch costs no memory round trip
+// of its own.  Once tr(X_{k-1} - X_{k-1}^2) < tol, step k is the last one
+// (the stop word records k); steps after the recorded one store nothing, so
+// the host may queue more steps than turn out to be needed.  Such a step
+// still loads and multiplies (nothing waits on the stop word before the
+// stores: an early exit there made the compiler wait for it before issuing
+// the operand loads).
+template <bool DIRECT, int C>
+__device__ __forceinline__ void tc2_step_body(const Tc2Step& a, double (&red)[TC2_W][4][4][64])
+{
+    // upper tile (bi, bj), bi <= bj, from the linear block index
+    int t = blockIdx.x, bi = 0;
+    while (t >= a.nt - bi) {
+        t -= a.nt - bi;
+        ++bi;
+    }
+    const int bj = bi + t;
+    const int i0 = bi * TC2_T, j0 = bj * TC2_T;
+    const int ld = a.ld;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lq = lane >> 4;
+    // the block's own tile of X_k (and its mirror) is written at the end;
+    // its elements are loaded now, with the operands
+    constexpr int NX = TC2_T * TC2_T / (64 * TC2_W);
+    double xp[NX], xq[NX];
+#pragma unroll
+    for (int n = 0; n < NX; ++n) {
+        const int e = tid + 64 * TC2_W * n;
+        const size_t idx = (size_t)(i0 + (e >> 5)) * ld + j0 + (e & 31);
+        xp[n] = a.P[idx];
+        xq[n] = DIRECT ? 0.0 : a.Q[idx];
+    }
+    // the step's inputs (written by earlier launches: plain loads)
+    int stop = 0x7fffffff;
+    double t1 = 0.0, t2 = 0.0;
+    if (!DIRECT) {
+        stop = *a.stop;
+        t1 = a.tin[0];
+        t2 = a.tin[1];
+    }
+    const int nkb = ld / 16;
+    Tc2Round<DIRECT, C> rd;
+    rd.load(a, wave, nkb, i0, j0, li, lq);
+    double cp = 1.0, cq = 0.0;   // X_k = cp P + cq Q
+    bool active = true;          // uniform over the grid
+    if (!DIRECT) {
+        active = !(stop < a.k);
+        if (active && t1 - t2 < a.tol && blockIdx.x == 0 && threadIdx.x == 0) *a.stop = a.k;
+        const bool up = fabs(t2 - a.nocc) < fabs(2.0 * t1 - t2 - a.nocc);
+        cp = up ? 0.0 : 2.0;
+        cq = up ? 1.0 : -1.0;
+    }
+    tc2_d4 acc[2][2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) acc[u][v] = (tc2_d4){0.0, 0.0, 0.0, 0.0};
+    rd.mul(cp, cq, acc);
+    for (int kb0 = wave + TC2_W * C; kb0 < nkb; kb0 += TC2_W * C) {   // more rounds (ld > 128 C)
+        rd.load(a, kb0, nkb, i0, j0, li, lq);
+        rd.mul(cp, cq, acc);
+    }
 #pragma unroll
     for (int u = 0; u < 2; ++u)
 #pragma unroll
@@ -206,6 +244,7 @@ __device__ __forceinline__ void tc2_step_body(const Tc2Step& a, bool up, double 
 #pragma unroll
             for (int r = 0; r < 4; ++r) red[wave][2 * u + v][r][lane] = acc[u][v][r];
     __syncthreads();
+    if (!active) return;
     // output slot e = (sub-tile g, register r, lane l), 1024 of them, summed
     // over the waves; sub-tile g: u = g / 2, v = g % 2
     double trx = 0.0, trx2 = 0.0;
@@ -238,21 +277,20 @@ __device__ __forceinline__ void tc2_step_body(const Tc2Step& a, bool up, double 
     }
 }
 
-// Step k.  Once tr(X_{k-1} - X_{k-1}^2) < tol, step k is the last one (the
-// stop word records k); steps after the recorded one return at once, so the
-// host may queue more steps than turn out to be needed.
+// C = k blocks of 16 per wave and round: ceil(ld / (16 TC2_W)), at most 4.
+// EARLY: read the stop word first and return at once past the recorded last
+// step (a queued step that is most likely not needed then costs only its
+// launch; when it is needed it pays one more memory round trip).
+template <int C, bool EARLY>
 __global__ void __launch_bounds__(64 * TC2_W) tc2_step_kernel(const Tc2Step a)
 {
     __shared__ double red[TC2_W][4][4][64];   // [wave][sub-tile][register][lane]
     if (a.direct) {
-        tc2_step_body<true>(a, true, red);
+        tc2_step_body<true, C>(a, red);
         return;
     }
-    const int stop = *(volatile const int*)a.stop;
-    const double t1 = a.tin[0], t2 = a.tin[1];
-    if (stop < a.k) return;
-    if (t1 - t2 < a.tol && blockIdx.x == 0 && threadIdx.x == 0) *a.stop = a.k;
-    tc2_step_body<false>(a, fabs(t2 - a.nocc) < fabs(2.0 * t1 - t2 - a.nocc), red);
+    if (EARLY && *a.stop < a.k) return;
+    tc2_step_body<false, C>(a, red);
 }
 
 // P = sym(X_n) (m x m): n - 1 = the recorded last step, or the last queued
@@ -277,6 +315,8 @@ __global__ void tc2_finish_kernel(const double* __restrict__ pair0, const double
 }
 
 static inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
+// k blocks of 16 per wave and round of the step kernel
+static inline int tc2_rounds(int ld) { return std::min(4, std::max(1, (ld / 16 + TC2_W - 1) / TC2_W)); }
 
 static inline double* tc2_X(const Tc2Work& w, int k) { return w.pair[k & 1]; }
 static inline double* tc2_X2(const Tc2Work& w, int k) { return w.pair[k & 1] + (size_t)w.ld * w.ld; }
@@ -313,7 +353,7 @@ int tc2_start(const double* F, Tc2Work& w, hipStream_t st)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int tc2_steps(const Tc2Work& w, int nocc, double tol, int k0, int steps, hipStream_t st)
+int tc2_steps(const Tc2Work& w, int nocc, double tol, int k0, int steps, hipStream_t st, bool early)
 {
     if (k0 < 0 || steps < 0 || k0 + steps > w.max_steps) return -1;
     const int nt = w.ld / TC2_T;
@@ -332,7 +372,19 @@ int tc2_steps(const Tc2Work& w, int nocc, double tol, int k0, int steps, hipStre
         a.tol = tol;
         a.ld = w.ld;
         a.nt = nt;
-        hipLaunchKernelGGL(tc2_step_kernel, dim3(nt * (nt + 1) / 2), dim3(64 * TC2_W), 0, st, a);
+        const dim3 grid(nt * (nt + 1) / 2), block(64 * TC2_W);
+        const int c = tc2_rounds(w.ld);
+        if (early) {
+            if (c == 1) hipLaunchKernelGGL((tc2_step_kernel<1, true>), grid, block, 0, st, a);
+            else if (c == 2) hipLaunchKernelGGL((tc2_step_kernel<2, true>), grid, block, 0, st, a);
+            else if (c == 3) hipLaunchKernelGGL((tc2_step_kernel<3, true>), grid, block, 0, st, a);
+            else hipLaunchKernelGGL((tc2_step_kernel<4, true>), grid, block, 0, st, a);
+        } else {
+            if (c == 1) hipLaunchKernelGGL((tc2_step_kernel<1, false>), grid, block, 0, st, a);
+            else if (c == 2) hipLaunchKernelGGL((tc2_step_kernel<2, false>), grid, block, 0, st, a);
+            else if (c == 3) hipLaunchKernelGGL((tc2_step_kernel<3, false>), grid, block, 0, st, a);
+            else hipLaunchKernelGGL((tc2_step_kernel<4, false>), grid, block, 0, st, a);
+        }
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -79,6 +80,87 @@ __global__ void __launch_bounds__(256) scf_diag_kernel(double* __restrict__ P, i
     P[e] = (e / m == e % m) ? d : 0.0;
 }
 
+// C (M x N, ldc) = op(A) op(B) for the few-hundred-row matrices of the SCF
+// (FP64, v_mfma_f64_16x16x4f64).  One block of eight waves per 32 x 32
+// output tile; each wave sums its share of k (16-wide k blocks kb = wave,
+// wave + 8, ...) straight from global memory, with all of its loads in
+// flight at once, and one LDS pass adds the eight partial tiles.  MFMA step
+// s pairs lane group q with k = 16 kb + 4 q + s; lane l supplies
+// A[l%16][k] and B[k][l%16], result register r of lane l is
+// D[4r + l/16][l%16].  Indices are clamped into the matrices and the
+// values outside masked to zero.  op(A)(i, k) = TA ? A[k lda + i] :
+// A[i lda + k]; op(B)(k, j) = TB ? B[j ldb + k] : B[k ldb + j].
+struct SmallGemm {
+    const double* A;
+    const double* B;
+    double* C;
+    int lda, ldb, ldc;
+    int M, N, K;
+};
+
+#define SG_W 8
+
+template <bool TA, bool TB, int CK>
+__global__ void __launch_bounds__(64 * SG_W) small_gemm_kernel(const SmallGemm g)
+{
+    typedef double sg_d4 __attribute__((ext_vector_type(4)));
+    __shared__ double red[SG_W][4][4][64];
+    const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lq = lane >> 4;
+    const int M = g.M, N = g.N, K = g.K;
+    const int nkb = (K + 15) / 16;
+    sg_d4 acc[2][2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) acc[u][v] = (sg_d4){0.0, 0.0, 0.0, 0.0};
+    for (int kb0 = wave; kb0 < nkb; kb0 += SG_W * CK) {
+        double a[CK][2][4], b[CK][2][4];
+#pragma unroll
+        for (int c = 0; c < CK; ++c)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int k = 16 * (kb0 + c * SG_W) + 4 * lq + s;
+                const bool kin = k < K;
+                const int kc = min(k, K - 1);
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int i = i0 + 16 * u + li, ic = min(i, M - 1);
+                    const double x = TA ? g.A[(long long)kc * g.lda + ic] : g.A[(long long)ic * g.lda + kc];
+                    a[c][u][s] = (kin && i < M) ? x : 0.0;
+                    const int j = j0 + 16 * u + li, jc = min(j, N - 1);
+                    const double y = TB ? g.B[(long long)jc * g.ldb + kc] : g.B[(long long)kc * g.ldb + jc];
+                    b[c][u][s] = (kin && j < N) ? y : 0.0;
+                }
+            }
+#pragma unroll
+        for (int c = 0; c < CK; ++c)
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int v = 0; v < 2; ++v)
+                        acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c][u][s], b[c][v][s], acc[u][v], 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) red[wave][2 * u + v][r][lane] = acc[u][v][r];
+    __syncthreads();
+    for (int e = tid; e < 1024; e += 64 * SG_W) {
+        const int gq = e >> 8, r = (e >> 6) & 3, l = e & 63, u = gq >> 1, v = gq & 1;
+        const int i = i0 + 16 * u + 4 * r + (l >> 4), j = j0 + 16 * v + (l & 15);
+        if (i >= M || j >= N) continue;
+        double x = 0.0;
+#pragma unroll
+        for (int w = 0; w < SG_W; ++w) x += red[w][gq][r][l];
+        g.C[(long long)i * g.ldc + j] = x;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -101,12 +183,31 @@ int bad(const std::string& m)
 
 inline unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
 
+template <bool TA, bool TB>
+void small_gemm_launch(const SmallGemm& g, hipStream_t st)
+{
+    const dim3 grid((g.N + 31) / 32, (g.M + 31) / 32), block(64 * SG_W);
+    const int nkb = (g.K + 15) / 16, ck = std::min(4, std::max(1, (nkb + SG_W - 1) / SG_W));
+    switch (ck) {
+    case 1: hipLaunchKernelGGL((small_gemm_kernel<TA, TB, 1>), grid, block, 0, st, g); break;
+    case 2: hipLaunchKernelGGL((small_gemm_kernel<TA, TB, 2>), grid, block, 0, st, g); break;
+    case 3: hipLaunchKernelGGL((small_gemm_kernel<TA, TB, 3>), grid, block, 0, st, g); break;
+    default: hipLaunchKernelGGL((small_gemm_kernel<TA, TB, 4>), grid, block, 0, st, g); break;
+    }
+}
+
 // C (M x N, ldc) = op(A) op(B): ta = A stored K x M, tb = B stored N x K.
 int gemm(int ta, int tb, int M, int N, int K, const double* A, long long lda, const double* B, long long ldb,
          double* C, long long ldc, hipStream_t st)
 {
-    GemmF64Args g{A, B, C, lda, ldb, ldc, 0, 0, 0, M, N, K, ta, tb, 1.0, 0.0};
-    if (gemm_f64_launch(g, 1, st)) return bad("pq_scf_run: GEMM launch failed");
+    if (M <= 0 || N <= 0 || K <= 0) return bad("pq_scf_run: empty GEMM");
+    if ((long long)M * N > (1LL << 30) || K > (1 << 24)) return bad("pq_scf_run: GEMM too large");
+    const SmallGemm g{A, B, C, (int)lda, (int)ldb, (int)ldc, M, N, K};
+    if (ta && tb) small_gemm_launch<true, true>(g, st);
+    else if (ta) small_gemm_launch<true, false>(g, st);
+    else if (tb) small_gemm_launch<false, true>(g, st);
+    else small_gemm_launch<false, false>(g, st);
+    if (hipGetLastError() != hipSuccess) return bad("pq_scf_run: GEMM launch failed");
     return 0;
 }
 
@@ -133,7 +234,7 @@ struct Bufs {
     double *P[2], *Fx[2];
     // temporaries
     double *T1, *A, *SD, *B;
-    double *pur, *dots;
+    double *pur[2], *dots;
 };
 
 long long lay_out(Bufs& b, double* base, int n, int nx, int nc, int nsp, int nslot)
@@ -166,7 +267,8 @@ long long lay_out(Bufs& b, double* base, int n, int nx, int nc, int nsp, int nsl
     b.A = c.take(big);
     b.SD = c.take(big);
     b.B = c.take(big);
-    b.pur = c.take(tc2_work_doubles(nx, SCF_MAX_STEPS));
+    b.pur[0] = c.take(tc2_work_doubles(nx, SCF_MAX_STEPS));
+    b.pur[1] = nsp > 1 ? c.take(tc2_work_doubles(nx, SCF_MAX_STEPS)) : nullptr;
     b.dots = c.take(scf_dots_work(n2, SCF_MAXDOT + 4));
     return c.used;
 }
@@ -351,6 +453,67 @@ extern "C" int pq_scf_run(pq_basis* basis, const double* H, const double* S, con
     int it = 0, steps_total = 0;
     const double gtol = sqrt(conv);
 
+    // Host-visible words: the purifications' stop words and the cycle's
+    // scalars, copied asynchronously (pinned memory, one block per thread,
+    // kept for the process).
+    static thread_local double* pin = nullptr;
+    if (!pin) SCFCHK(hipHostMalloc((void**)&pin, 64 * sizeof(double), hipHostMallocDefault));
+    int* pstop = (int*)pin;       // [2]
+    double* pvals = pin + 8;      // [SCF_MAXDOT + 4]
+    // Purification without a host round trip: the step count of the spin's
+    // last purification (the first: first_guess) on the fast kernel, 4 more on the
+    // early-exit kernel, the finish and the new density, all queued; the
+    // stop word is checked at the next synchronisation (check_purify).
+    const double ptol = 1e-11;
+    int first_guess = 44;   // PQ_SCF_PURIFY_START overrides it (tests of the continuation path)
+    if (const char* e = getenv("PQ_SCF_PURIFY_START")) first_guess = std::max(1, atoi(e));
+    int usual[2] = {first_guess, first_guess}, queued[2] = {0, 0};
+    bool pending[2] = {false, false};
+    auto density = [&](int s) -> int {   // D_s = X P_s X^T
+        SCFCALL(gemm(0, 0, n, nx, nx, X, nx, b.P[s], nx, b.T1, nx, st));   // X P
+        return gemm(0, 1, n, n, nx, b.T1, nx, X, nx, D[s], n, st);         // (X P) X^T
+    };
+    auto queue_purify = [&](int s, const double* Fuse) -> int {
+        Tc2Work w = tc2_work(b.pur[s], nx, SCF_MAX_STEPS);
+        if (tc2_start(Fuse, w, st)) return bad("pq_scf_run: purification launch failed");
+        const int fast = std::min(SCF_MAX_STEPS, std::max(1, usual[s]));
+        const int extra = std::min(SCF_MAX_STEPS - fast, 4);
+        if (tc2_steps(w, nocc[s], ptol, 0, fast, st, false) ||
+            (extra > 0 && tc2_steps(w, nocc[s], ptol, fast, extra, st, true)))
+            return bad("pq_scf_run: purification launch failed");
+        queued[s] = fast + extra;
+        if (tc2_finish(w, nocc[s], queued[s], b.P[s], st)) return bad("pq_scf_run: purification launch failed");
+        SCFCHK(hipMemcpyAsync(pstop + s, w.stop, sizeof(int), hipMemcpyDeviceToHost, st));
+        pending[s] = true;
+        return 0;
+    };
+    // After a synchronisation: a purification that had not stopped within
+    // its queued steps continues here (host-checked, 16 steps at a time),
+    // then P_s and D_s are formed again and `redo` is set.
+    auto check_purify = [&](int s, bool& redo) -> int {
+        if (!pending[s]) return 0;
+        pending[s] = false;
+        int stop = pstop[s], q = queued[s];
+        if (stop >= q && q < SCF_MAX_STEPS) {
+            const Tc2Work w = tc2_work(b.pur[s], nx, SCF_MAX_STEPS);
+            while (stop >= q && q < SCF_MAX_STEPS) {
+                const int more = std::min(16, SCF_MAX_STEPS - q);
+                if (tc2_steps(w, nocc[s], ptol, q, more, st, true)) return bad("pq_scf_run: purification launch failed");
+                q += more;
+                SCFCHK(hipMemcpyAsync(pstop + s, w.stop, sizeof(int), hipMemcpyDeviceToHost, st));
+                SCFCHK(hipStreamSynchronize(st));
+                stop = pstop[s];
+            }
+            if (tc2_finish(w, nocc[s], q, b.P[s], st)) return bad("pq_scf_run: purification launch failed");
+            SCFCALL(density(s));
+            redo = true;
+        }
+        const int last = std::min(stop, q - 1);
+        usual[s] = last + 1;
+        steps_total += last + 1;
+        return 0;
+    };
+
     // Cartesian density -> J/K input; J, K -> final basis
     auto to_cart = [&](const double* Dn, double* Dc) -> int {   // Dc = T^T D T
         SCFCALL(gemm(0, 0, n, ncart, n, Dn, n, T, ncart, b.Tmp, ncart, st));       // D T (n x nc)
@@ -462,8 +625,30 @@ extern "C" int pq_scf_run(pq_basis* basis, const double* H, const double* S, con
             nd[s] = diis[s].k + 1;
         }
         for (int s = 0; s < nsp; ++s) rs.push_back(b.R[s][cur[s]]);
-        double vals[SCF_MAXDOT + 4];
-        SCFCALL(pq_scf_dots(xs.data(), ys.data(), (int)xs.size(), rs.data(), nsp, n2, b.dots, vals, st));
+        {
+            ScfDotArgs da = {};
+            for (size_t i = 0; i < xs.size(); ++i) {
+                da.x[i] = xs[i];
+                da.y[i] = ys[i];
+            }
+            for (int s = 0; s < nsp; ++s) da.r[s] = rs[s];
+            da.npair = (int)xs.size();
+            da.nr = nsp;
+            da.n = n2;
+            if (scf_dots(da, b.dots, st)) return bad("pq_scf_run: scalar reduction launch failed");
+            const int nout = da.npair + 2 * nsp;
+            SCFCHK(hipMemcpyAsync(pvals, b.dots + (size_t)scf_dots_blocks(n2) * nout, sizeof(double) * nout,
+                                  hipMemcpyDeviceToHost, st));
+            SCFCHK(hipStreamSynchronize(st));
+        }
+        const double* vals = pvals;
+        // the densities this cycle used: their purifications must have stopped
+        bool redo = false;
+        for (int s = 0; s < nsp; ++s) SCFCALL(check_purify(s, redo));
+        if (redo) {   // this cycle again, with the finished densities
+            --it;
+            continue;
+        }
         if (jk_ms && !unrestricted) {   // this cycle's J/K has finished: its events are complete
             double ms = -1.0;
             if (pq_get_timings(basis, nullptr, &ms, nullptr) == 0) jk_ms[2 * (it - 1)] = ms;
@@ -528,13 +713,9 @@ extern "C" int pq_scf_run(pq_basis* basis, const double* H, const double* S, con
                                    nocc[s] == 0 ? 0.0 : 1.0);
                 SCFCHK(hipGetLastError());
             } else {
-                int steps = 0;
-                double idem = 0.0;
-                SCFCALL(pq_purify(Fuse, nx, nocc[s], b.P[s], b.pur, 1e-11, SCF_MAX_STEPS, &steps, &idem, st));
-                steps_total += steps;
+                SCFCALL(queue_purify(s, Fuse));
             }
-            SCFCALL(gemm(0, 0, n, nx, nx, X, nx, b.P[s], nx, b.T1, nx, st));   // X P
-            SCFCALL(gemm(0, 1, n, n, nx, b.T1, nx, X, nx, D[s], n, st));       // (X P) X^T
+            SCFCALL(density(s));
             have_p[s] = true;
         }
         if (it == maxiter) {   // the loop ends here: hand back the (extrapolated) Fock matrices
@@ -550,6 +731,11 @@ extern "C" int pq_scf_run(pq_basis* basis, const double* H, const double* S, con
                 }
             }
         }
+    }
+    if (pending[0] || pending[1]) {   // stopped after maxiter cycles: finish the last purifications
+        SCFCHK(hipStreamSynchronize(st));
+        bool redo = false;
+        for (int s = 0; s < nsp; ++s) SCFCALL(check_purify(s, redo));
     }
     if (!unrestricted && Db && Db != Da) SCFCHK(hipMemcpyAsync(Db, Da, sizeof(double) * n2, hipMemcpyDeviceToDevice, st));
     if (!unrestricted && Fb && Fb != Fa) SCFCHK(hipMemcpyAsync(Fb, Fa, sizeof(double) * n2, hipMemcpyDeviceToDevice, st));

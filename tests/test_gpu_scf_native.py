@@ -107,3 +107,21 @@ def test_scf_run_rejects_bad_arguments(gpu):
         eng.scf_run(H, H, X, H, H, 5, 4, False, 1e-8, 10)        # restricted needs nalpha == nbeta
     with pytest.raises(_lib.HipError):
         eng.scf_run(H, H, X, H, H, 5, 5, True, 1e-8, 10, diis_size=10)
+
+
+@pytest.mark.parametrize("case", ["propane_631gs_6d_sad", "h3_doublet_uhf_core"])
+def test_native_loop_purification_continuation(gpu, monkeypatch, case):
+    """The first purification predicted at 5 steps (PQ_SCF_PURIFY_START) has
+    not stopped at the next synchronisation: the loop finishes it, forms the
+    density again and repeats that cycle; the result is the same."""
+    from pychem_amd import hartree_fock
+    make, guess = CASES[case]
+    r1 = hartree_fock.do(make(), conv=1e-10, maxiter=200, guess=guess)
+    monkeypatch.setenv("PQ_SCF_PURIFY_START", "5")
+    r2 = hartree_fock.do(make(), conv=1e-10, maxiter=200, guess=guess)
+    assert r1["converged"] and r2["converged"]
+    assert abs(r1["energy"] - r2["energy"]) < 1e-9
+    # the J/K reduction's atomics may round differently between runs: a
+    # long (UHF) iteration may then take a few cycles more or less
+    assert abs(r1["iterations"] - r2["iterations"]) <= max(1, r1["iterations"] // 6)
+    assert np.allclose(r1["history"][:3], r2["history"][:3], rtol=0, atol=1e-9)
