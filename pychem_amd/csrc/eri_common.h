@@ -61,6 +61,13 @@ struct OneeArgs {
     double *S, *T, *V;       // packed per-molecule blocks
     int npair;
 };
+// the six pair classes in one launch: class p covers blocks [block0[p],
+// block0[p + 1]) of 128 threads, G lanes per shell pair (G: power of two <= 64)
+struct OneeFusedArgs {
+    OneeArgs cls[6];
+    int block0[7];
+    int G;
+};
 
 template <typename real>
 struct JkArgs {

@@ -202,7 +202,11 @@ def gen_eri_class(la, lb, lc, ld):
 
 
 def gen_onee(la, lb):
-    """S, T, V for one pair class; thread per shell pair."""
+    """S, T, V for one pair class as a device function for the fused launch:
+    G consecutive lanes (a power of two <= 64, aligned) share one shell pair;
+    lane g sums the nuclear attraction over atoms g, g + G, ... of the pair's
+    molecule, the lanes combine with shuffles, and lane 0 also forms S and T
+    and writes the three blocks."""
     tag = "%d%d" % (la, lb)
     es = Emitter(indent="    ")
 
@@ -288,10 +292,9 @@ def gen_onee(la, lb):
     vbody = ev.lines
 
     lines = []
-    lines.append("__global__ void __launch_bounds__(128) onee_k%s(const OneeArgs args)" % tag)
+    lines.append("__device__ __forceinline__ void onee_body_%s(const OneeArgs& args, int t, int g, int G)" % tag)
     lines.append("{")
-    lines.append("  const int t = blockIdx.x * blockDim.x + threadIdx.x;")
-    lines.append("  if (t >= args.npair) return;")
+    lines.append("  if (t >= args.npair) return;   // uniform over the pair's lanes")
     lines.append("  const int pidx = args.pair_list[t];")
     lines.append("  const PairHdr h = args.pairs[pidx];")
     lines.append("  const MolInfo mi = args.mols[h.mol];")
@@ -304,8 +307,10 @@ def gen_onee(la, lb):
     lines.append("    const PrimPair<double> P = args.prims[h.off + i];")
     lines.append("    const double PAx = P.PAx, PAy = P.PAy, PAz = P.PAz;")
     lines.append("    const double beta = P.beta;")
-    lines.extend(sbody)
-    lines.append("    for (int c = mi.atom_lo; c < mi.atom_hi; ++c) {")
+    lines.append("    if (g == 0) {")
+    lines.extend(["  " + x for x in sbody])
+    lines.append("    }")
+    lines.append("    for (int c = mi.atom_lo + g; c < mi.atom_hi; c += G) {")
     lines.append("      const double PCx = P.Px - args.atoms[4 * c], PCy = P.Py - args.atoms[4 * c + 1], PCz = P.Pz - args.atoms[4 * c + 2];")
     lines.append("      const double Tn = P.p * (PCx * PCx + PCy * PCy + PCz * PCz);")
     lines.append("      double F[%d];" % (Lv + 1))
@@ -314,6 +319,12 @@ def gen_onee(la, lb):
     lines.extend(vbody)
     lines.append("    }")
     lines.append("  }")
+    lines.append("  for (int o = G >> 1; o > 0; o >>= 1) {")
+    for a in comps(la):
+        for b in comps(lb):
+            lines.append("    v_%s_%s += __shfl_xor(v_%s_%s, o, 64);" % ((nm(a), nm(b)) * 2))
+    lines.append("  }")
+    lines.append("  if (g != 0) return;")
     lines.append("  const int o1 = args.cart_off[h.s1], o2 = args.cart_off[h.s2];")
     lines.append("  const int N = mi.n;")
     lines.append("  double* const Sm = args.S + mi.base;")
@@ -432,15 +443,22 @@ def main():
     onee = ["// generated by gen_eri.py -- do not edit", '#include "../eri_common.h"']
     for la, lb in PAIRS:
         onee.append(gen_onee(la, lb))
-    onee.append("void onee_run(int pcls, const OneeArgs& a, hipStream_t st)")
+    onee.append("// every pair class in one launch: block -> class by the first-block table")
+    onee.append("__global__ void __launch_bounds__(128) onee_fused_k(const OneeFusedArgs a)")
     onee.append("{")
-    onee.append("  const dim3 grid((a.npair + 127) / 128), block(128);")
-    onee.append("  switch (pcls) {")
+    onee.append("  const int b = blockIdx.x;")
+    onee.append("  int j = 0;")
+    onee.append("  while (j + 1 < 6 && b >= a.block0[j + 1]) ++j;")
+    onee.append("  const int lt = (b - a.block0[j]) * 128 + threadIdx.x;")
+    onee.append("  const int t = lt / a.G, g = lt - (lt / a.G) * a.G;")
+    onee.append("  switch (j) {")
     for p, (la, lb) in enumerate(PAIRS):
-        onee.append("  case %d: hipLaunchKernelGGL(onee_k%d%d, grid, block, 0, st, a); break;" % (p, la, lb))
+        onee.append("  case %d: onee_body_%d%d(a.cls[%d], t, g, a.G); break;" % (p, la, lb, p))
     onee.append("  default: break;")
     onee.append("  }")
     onee.append("}")
+    onee.append("void onee_fused_run(const OneeFusedArgs& a, hipStream_t st)")
+    onee.append("{ if (a.block0[6] > 0) hipLaunchKernelGGL(onee_fused_k, dim3(a.block0[6]), dim3(128), 0, st, a); }")
     _write_if_changed(os.path.join(gdir, "onee.hip"), "\n".join(onee) + "\n")
     # dispatch header
     d = ["// generated by gen_eri.py -- do not edit", "#pragma once", '#include "jk_common.h"',
@@ -452,7 +470,7 @@ def main():
             d.append("size_t eri_scratch_%s_%s();" % (tag, rn))
             d.append("void jk_run_%s_%s(const JkArgs<%s>&, hipStream_t);" % (tag, rn, rt))
             d.append("void scatter_run_%s_%s(const ScatterArgs<%s>&, hipStream_t);" % (tag, rn, rt))
-    d.append("void onee_run(int pcls, const OneeArgs& a, hipStream_t st);")
+    d.append("void onee_fused_run(const OneeFusedArgs& a, hipStream_t st);")
     d.append("// classes of the fused heavy-class launch: slot in eri_fused_k, or -1")
     d.append("#define ERI_NFUSED %d" % len(FUSED))
     d.append("static const int eri_fused_slot[ERI_NCLASS] = {")

@@ -50,6 +50,46 @@ void list_items_count(const ListBras& b64, const ListBras* b32, int nb, long lon
 void list_fill(const ListArgs& a, const ListBras& b, long long nq, int2* quartets, unsigned* kets, hipStream_t st);
 void list_items_fill(const ListArgs& a, const ListBras& b64, const ListBras* b32, long long nitems, long long run,
                      JkItem* items, int2* items32, hipStream_t st);
+// Batched construction: one launch per phase over every class of a plan.
+// A job is one ERI class: its bra/ket lists and both (FP64, FP32) quartet
+// lists.  Work is laid out job after job in whole blocks of 256 threads;
+// block0[j] is job j's first block (block0[n] = all blocks), so the job of a
+// block is found with scalar loads from the kernel arguments.
+#define LIST_MAXJOB 48
+struct ListJob {
+    ListArgs a;            // a.nb = 0: nothing to count for this class
+    ListBras b64, b32;
+    int2* q[2];            // quartet lists (FP64, FP32)
+    unsigned* k[2];        // ket records
+    long long nq[2];
+    JkItem* items;
+    int2* items32;         // NULL without an FP32 list
+    long long nitems;
+    long long run;         // work-item length
+    int nb_items;          // bras whose items are counted (0: class without quartets)
+    int has32;             // the class has an FP32 list
+};
+struct ListBlocks {
+    int block0[LIST_MAXJOB + 1];
+    int n;
+};
+struct ScanJob {
+    const long long* in;
+    long long* out;
+    int n;
+    long long* total;
+};
+// per bra of every job: list_count_kernel's runs (block0 over jobs' a.nb)
+void list_count_batch(const ListJob* jobs, const ListBlocks& blk, hipStream_t st);
+// one single-workgroup exclusive scan per ScanJob (jobs in device memory)
+void list_scan_batch(const ScanJob* jobs, int n, hipStream_t st);
+// every quartet of every list: jobs' list w = 0, 1 as blocks 2j + w
+void list_fill_batch(const ListJob* jobs, const ListBlocks& blk, hipStream_t st);
+// per bra of every job (block0 over nb_items): work items of its kets
+void list_items_count_batch(const ListJob* jobs, const ListBlocks& blk, hipStream_t st);
+// every work item of every job (block0 over nitems)
+void list_items_fill_batch(const ListJob* jobs, const ListBlocks& blk, hipStream_t st);
+
 // J'_Q += sum over the bras that reach ket Q of the per-quartet ket J values
 // jkbuf[(tile * ncd + cd) * nq + q], over tiles of up to 64 kets x 256 bras of
 // one molecule (int4: first ket, kets, first bra, bras); one list per launch,

@@ -36,10 +36,8 @@ __device__ __forceinline__ int last_le(const long long* v, int n, long long x)
 //         diagonal when bra and ket lists agree;
 //   f64 = first ket with bound_i bound_j < thr32 (thr32 > 0), clipped to
 //         [j0 + diagonal, fa].
-__global__ void list_count_kernel(const ListArgs a, const ListBras b64, const ListBras b32)
+__device__ __forceinline__ void list_count_one(const ListArgs& a, const ListBras& b64, const ListBras& b32, int i)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.nb) return;
     const int P = a.bra[i];
     const PairHdr ph = a.pairs[P];
     const double qb = a.bound[P];
@@ -59,9 +57,15 @@ __global__ void list_count_kernel(const ListArgs a, const ListBras b64, const Li
     b32.np[i] = npb * (a.ket_npcum[fa] - a.ket_npcum[f64]);
 }
 
+__global__ void list_count_kernel(const ListArgs a, const ListBras b64, const ListBras b32)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.nb) list_count_one(a, b64, b32, i);
+}
+
 // Single-workgroup exclusive scan: each thread scans a contiguous chunk,
 // chunk sums are scanned in LDS.
-__global__ void __launch_bounds__(1024) list_scan_kernel(const long long* in, long long* out, int n, long long* total)
+__device__ __forceinline__ void list_scan_one(const long long* in, long long* out, int n, long long* total)
 {
     __shared__ long long part[1024];
     const int t = threadIdx.x;
@@ -86,6 +90,17 @@ __global__ void __launch_bounds__(1024) list_scan_kernel(const long long* in, lo
     if (t == 1023) *total = part[1023];
 }
 
+__global__ void __launch_bounds__(1024) list_scan_kernel(const long long* in, long long* out, int n, long long* total)
+{
+    list_scan_one(in, out, n, total);
+}
+
+__global__ void __launch_bounds__(1024) list_scan_batch_kernel(const ScanJob* jobs)
+{
+    const ScanJob j = jobs[blockIdx.x];
+    list_scan_one(j.in, j.out, j.n, j.total);
+}
+
 // Items of a class: per bra, its FP64 kets then its FP32 kets cut into
 // chunks of `run` quartets.
 __global__ void list_items_count_kernel(const long long* cnt64, const long long* cnt32, long long* nit, int nb,
@@ -97,10 +112,9 @@ __global__ void list_items_count_kernel(const long long* cnt64, const long long*
 
 // One thread per quartet: its bra is the last with qs <= q (bras without
 // kets share the next bra's qs and are never last).
-__global__ void list_fill_kernel(const ListArgs a, const ListBras b, long long nq, int2* quartets, unsigned* kets)
+__device__ __forceinline__ void list_fill_one(const ListArgs& a, const ListBras& b, long long q, int2* quartets,
+                                              unsigned* kets)
 {
-    const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (q >= nq) return;
     const int i = last_le(b.qs, a.nb, q);
     const int j = b.j0[i] + (int)(q - b.qs[i]);
     const int P = a.bra[i], Q = a.ket[j];
@@ -110,14 +124,18 @@ __global__ void list_fill_kernel(const ListArgs a, const ListBras b, long long n
               ((kh.s1 == kh.s2) ? (1u << 30) : 0u) | ((P == Q) ? (1u << 31) : 0u);
 }
 
+__global__ void list_fill_kernel(const ListArgs a, const ListBras b, long long nq, int2* quartets, unsigned* kets)
+{
+    const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (q < nq) list_fill_one(a, b, q, quartets, kets);
+}
+
 // Item t covers positions [lo, hi) of its bra's concatenated run (FP64
 // kets, then FP32 kets): FP64 quartets qs64 + [min(lo, c64), min(hi, c64)),
 // FP32 quartets qs32 + [max(lo - c64, 0), max(hi - c64, 0)).
-__global__ void list_items_fill_kernel(const ListArgs a, const ListBras b, const ListBras b32, long long nitems,
-                                       long long run, JkItem* items, int2* items32)
+__device__ __forceinline__ void list_items_fill_one(const ListArgs& a, const ListBras& b, const ListBras& b32,
+                                                    long long t, long long run, JkItem* items, int2* items32)
 {
-    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= nitems) return;
     const int i = last_le(b.its, a.nb, t);
     const long long r = t - b.its[i];
     const long long c64 = b.cnt[i], c32 = items32 ? b32.cnt[i] : 0;
@@ -133,6 +151,56 @@ __global__ void list_items_fill_kernel(const ListArgs a, const ListBras b, const
         const long long s0 = lo > c64 ? lo - c64 : 0, s1 = hi > c64 ? hi - c64 : 0;
         items32[t] = make_int2((int)(b32.qs[i] + s0), (int)(b32.qs[i] + s1));
     }
+}
+
+__global__ void list_items_fill_kernel(const ListArgs a, const ListBras b, const ListBras b32, long long nitems,
+                                       long long run, JkItem* items, int2* items32)
+{
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t < nitems) list_items_fill_one(a, b, b32, t, run, items, items32);
+}
+
+// ---- batched phases: job of a block from the first-block table (uniform)
+__device__ __forceinline__ int list_job_of(const ListBlocks& blk, int& local)
+{
+    const int b = blockIdx.x;
+    int j = 0;
+    while (j + 1 < blk.n && blk.block0[j + 1] <= b) ++j;
+    local = (b - blk.block0[j]) * 256 + threadIdx.x;
+    return j;
+}
+
+__global__ void __launch_bounds__(256) list_count_batch_kernel(const ListJob* jobs, const ListBlocks blk)
+{
+    int i;
+    const int j = list_job_of(blk, i);
+    const ListJob& J = jobs[j];
+    if (i < J.a.nb) list_count_one(J.a, J.b64, J.b32, i);
+}
+
+__global__ void __launch_bounds__(256) list_fill_batch_kernel(const ListJob* jobs, const ListBlocks blk)
+{
+    int q;
+    const int jw = list_job_of(blk, q), j = jw >> 1, w = jw & 1;
+    const ListJob& J = jobs[j];
+    if (q < J.nq[w]) list_fill_one(J.a, w ? J.b32 : J.b64, q, J.q[w], J.k[w]);
+}
+
+__global__ void __launch_bounds__(256) list_items_count_batch_kernel(const ListJob* jobs, const ListBlocks blk)
+{
+    int i;
+    const int j = list_job_of(blk, i);
+    const ListJob& J = jobs[j];
+    if (i < J.nb_items)
+        J.b64.nit[i] = (J.b64.cnt[i] + (J.has32 ? J.b32.cnt[i] : 0) + J.run - 1) / J.run;
+}
+
+__global__ void __launch_bounds__(256) list_items_fill_batch_kernel(const ListJob* jobs, const ListBlocks blk)
+{
+    int t;
+    const int j = list_job_of(blk, t);
+    const ListJob& J = jobs[j];
+    if (t < J.nitems) list_items_fill_one(J.a, J.b64, J.has32 ? J.b32 : J.b64, t, J.run, J.items, J.items32);
 }
 
 // Ket J reduction over tiles of up to 64 consecutive kets (one per lane) x up
@@ -263,6 +331,35 @@ void list_items_fill(const ListArgs& a, const ListBras& b64, const ListBras* b32
     if (nitems > 0)
         hipLaunchKernelGGL(list_items_fill_kernel, dim3(blocks_of(nitems, 256)), dim3(256), 0, st, a, b64,
                            b32 ? *b32 : b64, nitems, run, items, b32 ? items32 : (int2*)nullptr);
+}
+
+void list_count_batch(const ListJob* jobs, const ListBlocks& blk, hipStream_t st)
+{
+    if (blk.n > 0 && blk.block0[blk.n] > 0)
+        hipLaunchKernelGGL(list_count_batch_kernel, dim3(blk.block0[blk.n]), dim3(256), 0, st, jobs, blk);
+}
+
+void list_scan_batch(const ScanJob* jobs, int n, hipStream_t st)
+{
+    if (n > 0) hipLaunchKernelGGL(list_scan_batch_kernel, dim3(n), dim3(1024), 0, st, jobs);
+}
+
+void list_fill_batch(const ListJob* jobs, const ListBlocks& blk, hipStream_t st)
+{
+    if (blk.n > 0 && blk.block0[blk.n] > 0)
+        hipLaunchKernelGGL(list_fill_batch_kernel, dim3(blk.block0[blk.n]), dim3(256), 0, st, jobs, blk);
+}
+
+void list_items_count_batch(const ListJob* jobs, const ListBlocks& blk, hipStream_t st)
+{
+    if (blk.n > 0 && blk.block0[blk.n] > 0)
+        hipLaunchKernelGGL(list_items_count_batch_kernel, dim3(blk.block0[blk.n]), dim3(256), 0, st, jobs, blk);
+}
+
+void list_items_fill_batch(const ListJob* jobs, const ListBlocks& blk, hipStream_t st)
+{
+    if (blk.n > 0 && blk.block0[blk.n] > 0)
+        hipLaunchKernelGGL(list_items_fill_batch_kernel, dim3(blk.block0[blk.n]), dim3(256), 0, st, jobs, blk);
 }
 
 // Tiles of one list: per molecule, runs of up to 64 kets x 256 bras; with

@@ -323,8 +323,8 @@ struct DevBuf {
 };
 
 struct EriClassStore {
-    DevBuf q, vals, items, items32, kets, jkbuf, jtiles;   // items, items32: class level, in store64
-    std::vector<int4> hjtiles; // host copy of the ket J reduction tiles
+    DevBuf q, vals, items, items32, kets, jkbuf;   // items, items32: class level, in store64
+    const int4* jtiles = nullptr;   // the list's ket J reduction tiles (in the basis's d_jtiles)
     int njtile = 0;
     // per bra of the class's bra list (see lists.h)
     DevBuf j0, cnt, np, qs, nit, its;
@@ -357,6 +357,12 @@ struct pq_basis {
     std::vector<int> hseg[6], hsorted[6];    // host copies of what the device lists read
     std::vector<double> hsbound[6];
     std::vector<long long> hnpcum[6];
+    // batched list construction: job tables (device) and the host copies of
+    // every upload of the current plan (kept until the next plan)
+    DevBuf d_jobs, d_scans, d_jtiles;
+    std::vector<std::vector<ListJob>> hjobs;
+    std::vector<std::vector<ScanJob>> hscans;
+    std::vector<int4> hjtiles_all;
     // every tiled list's ket J reduction, launched once per J/K call
     DevBuf d_jred;
     std::vector<JredList> hjred;
@@ -376,8 +382,10 @@ struct pq_basis {
     hipEvent_t ev_pass[2] = {};   // whole ERI pass on the caller's stream
     int ev_pass_used = 0;
     hipGraphExec_t graph_exec = nullptr;
-    // J/K class launches fan out over side streams (default priority)
-    int jk_nstreams = 4;
+    // J/K class launches fan out over side streams (default priority).  Three
+    // by default: with the caller's stream that fills the process's four
+    // hardware queues (measured: C10H22 J/K 0.40 ms on 3, 0.46 ms on 4)
+    int jk_nstreams = 3;
     hipStream_t jk_side[4] = {};
     hipEvent_t ev_fork = nullptr, ev_join[4] = {};
 };
@@ -586,7 +594,7 @@ extern "C" void pq_basis_destroy(pq_basis* b)
         b->store32[c].q.release(); b->store32[c].vals.release(); b->store32[c].items.release();
         for (int w = 0; w < 2; ++w) {
             EriClassStore& t = w ? b->store32[c] : b->store64[c];
-            t.kets.release(); t.jkbuf.release(); t.jtiles.release(); t.j0.release(); t.cnt.release(); t.np.release();
+            t.kets.release(); t.jkbuf.release(); t.j0.release(); t.cnt.release(); t.np.release();
             t.qs.release(); t.nit.release(); t.its.release();
         }
     }
@@ -669,8 +677,18 @@ extern "C" int pq_one_electron(pq_basis* b, double* S, double* T, double* V, voi
     offs[6] = (int)all.size();
     if (b->d_tmp_list.ensure(sizeof(int) * std::max<size_t>(1, all.size()))) return -1;
     HIPCHK(hipMemcpyAsync(b->d_tmp_list.p, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, st));
+    // G lanes per shell pair split the nuclear attraction over the atoms of
+    // the pair's molecule: the most atoms of any molecule, rounded up to a
+    // power of two, at most 32
+    int amax = 1;
+    for (const MolInfo& mi : b->mols) amax = std::max(amax, mi.atom_hi - mi.atom_lo);
+    int G = 1;
+    while (G < amax && G < 32) G <<= 1;
+    OneeFusedArgs f = {};
+    f.G = G;
+    f.block0[0] = 0;
     for (int c = 0; c < 6; ++c) {
-        OneeArgs a;
+        OneeArgs& a = f.cls[c];
         a.pair_list = (const int*)b->d_tmp_list.p + offs[c];
         a.pairs = (const PairHdr*)b->d_pairs.p;
         a.prims = (const PrimPair<double>*)b->d_prims.p;
@@ -679,8 +697,9 @@ extern "C" int pq_one_electron(pq_basis* b, double* S, double* T, double* V, voi
         a.mols = (const MolInfo*)b->d_mols.p;
         a.S = S; a.T = T; a.V = V;
         a.npair = offs[c + 1] - offs[c];
-        if (a.npair > 0) onee_run(c, a, st);
+        f.block0[c + 1] = f.block0[c] + (int)(((long long)a.npair * G + 127) / 128);
     }
+    onee_fused_run(f, st);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     return 0;
@@ -969,80 +988,201 @@ extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold,
     if (upload_sorted_pairs(b, st)) return -1;
     if (b->d_total.ensure(sizeof(long long) * 4 * ERI_NCLASS)) return -1;
     long long* dtot = (long long*)b->d_total.p;
-    for (int c = 0; c < ERI_NCLASS; ++c) {
-        const ListArgs a = list_args(b, c);
-        const size_t nb = (size_t)std::max(1, a.nb);
-        for (int w = 0; w < 2; ++w) {
-            EriClassStore& s = w ? b->store32[c] : b->store64[c];
-            if (s.j0.ensure(sizeof(int) * nb) || s.cnt.ensure(8 * nb) || s.np.ensure(8 * nb) || s.qs.ensure(8 * nb) ||
-                s.nit.ensure(8 * nb) || s.its.ensure(8 * nb))
-                return -1;
+    // one launch per phase over every class (lists.h: batched construction);
+    // each upload's host copy stays in the handle until the next plan
+    b->hjobs.clear();
+    b->hscans.clear();
+    if (b->d_jobs.ensure(sizeof(ListJob) * ERI_NCLASS) || b->d_scans.ensure(sizeof(ScanJob) * 4 * ERI_NCLASS))
+        return -1;
+    auto upload_jobs = [&](const std::vector<ListJob>& v) -> int {
+        HIPCHK(hipMemcpyAsync(b->d_jobs.p, v.data(), sizeof(ListJob) * v.size(), hipMemcpyHostToDevice, st));
+        return 0;
+    };
+    auto upload_scans = [&](const std::vector<ScanJob>& v) -> int {
+        HIPCHK(hipMemcpyAsync(b->d_scans.p, v.data(), sizeof(ScanJob) * v.size(), hipMemcpyHostToDevice, st));
+        return 0;
+    };
+    // counts per bra, then their scans (totals: quartets and primitive
+    // quartets of both lists per class)
+    b->hjobs.emplace_back(ERI_NCLASS);
+    {
+        std::vector<ListJob>& jobs = b->hjobs.back();
+        ListBlocks blk = {};
+        blk.n = ERI_NCLASS;
+        for (int c = 0; c < ERI_NCLASS; ++c) {
+            ListArgs a = list_args(b, c);
+            const size_t nb = (size_t)std::max(1, a.nb);
+            for (int w = 0; w < 2; ++w) {
+                EriClassStore& s = w ? b->store32[c] : b->store64[c];
+                if (s.j0.ensure(sizeof(int) * nb) || s.cnt.ensure(8 * nb) || s.np.ensure(8 * nb) ||
+                    s.qs.ensure(8 * nb) || s.nit.ensure(8 * nb) || s.its.ensure(8 * nb))
+                    return -1;
+            }
+            if (a.nb == 0 || a.nk == 0) a.nb = 0;   // nothing kept: zero totals
+            jobs[c] = ListJob{};
+            jobs[c].a = a;
+            jobs[c].b64 = list_bras(b->store64[c]);
+            jobs[c].b32 = list_bras(b->store32[c]);
+            blk.block0[c + 1] = blk.block0[c] + (a.nb + 255) / 256;
         }
-        if (a.nb == 0 || a.nk == 0) {
-            hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(256), 0, st, (double*)(dtot + 4 * c), 4LL,
-                               (double*)nullptr);
-            continue;
+        if (upload_jobs(jobs)) return -1;
+        list_count_batch((const ListJob*)b->d_jobs.p, blk, st);
+        b->hscans.emplace_back();
+        std::vector<ScanJob>& sc = b->hscans.back();
+        for (int c = 0; c < ERI_NCLASS; ++c) {
+            const ListBras& l64 = jobs[c].b64;
+            const ListBras& l32 = jobs[c].b32;
+            const int n = jobs[c].a.nb;
+            sc.push_back({l64.cnt, l64.qs, n, dtot + 4 * c + 0});
+            sc.push_back({l32.cnt, l32.qs, n, dtot + 4 * c + 1});
+            sc.push_back({l64.np, l64.nit, n, dtot + 4 * c + 2});   // nit as scratch: only the totals are kept
+            sc.push_back({l32.np, l32.nit, n, dtot + 4 * c + 3});
         }
-        const ListBras l64 = list_bras(b->store64[c]), l32 = list_bras(b->store32[c]);
-        list_count(a, l64, l32, st);
-        list_scan(l64.cnt, l64.qs, a.nb, dtot + 4 * c + 0, st);
-        list_scan(l32.cnt, l32.qs, a.nb, dtot + 4 * c + 1, st);
-        list_scan(l64.np, l64.nit, a.nb, dtot + 4 * c + 2, st);   // nit as scratch: only the totals are kept
-        list_scan(l32.np, l32.nit, a.nb, dtot + 4 * c + 3, st);
+        if (upload_scans(sc)) return -1;
+        list_scan_batch((const ScanJob*)b->d_scans.p, (int)sc.size(), st);
     }
     HIPCHK(hipGetLastError());
     std::vector<long long> tot(4 * ERI_NCLASS);
     HIPCHK(hipMemcpyAsync(tot.data(), dtot, sizeof(long long) * tot.size(), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    for (int c = 0; c < ERI_NCLASS; ++c) {
-        const ListArgs a = list_args(b, c);
-        const int ncd = ncart_l(eri_class_table[c][2]) * ncart_l(eri_class_table[c][3]);
-        for (int w = 0; w < 2; ++w) {
-            EriClassStore& s = w ? b->store32[c] : b->store64[c];
-            const long long nq = tot[4 * c + w];
-            if (nq > 0x7fffffffLL) return fail("pq_eri_plan: more than 2^31 quartets in one class");
-            s.nq = (int)nq;
-            s.nprimq = tot[4 * c + 2 + w];
-            if (!s.nq) { s.nitems = 0; continue; }
-            const size_t vb = (size_t)s.nq * s.ncomp * (w ? sizeof(float) : sizeof(double));
-            const size_t jb = jk_tile_fits(b, c) ? sizeof(double) * (size_t)s.nq * jk_tile_table[c][2] * ncd : 0;
-            if (s.q.ensure(sizeof(int2) * s.nq) || s.vals.ensure(vb) || s.kets.ensure(sizeof(unsigned) * s.nq) ||
-                s.jkbuf.ensure(jb))
-                return -1;
-            const ListBras lb = list_bras(s);
-            list_fill(a, lb, s.nq, (int2*)s.q.p, (unsigned*)s.kets.p, st);
-        }
-        if (b->store64[c].nq || b->store32[c].nq) items_count(b, c, dtot + c, st);
+    // the lists themselves, then the work items' counts per bra and scans
+    {
+        std::vector<ListJob> next = b->hjobs.back();   // the next phase's table: a new host copy
+        b->hjobs.push_back(std::move(next));
     }
-    // every class's item count in one transfer
+    {
+        std::vector<ListJob>& jobs = b->hjobs.back();
+        ListBlocks fb = {};
+        fb.n = 2 * ERI_NCLASS;
+        for (int c = 0; c < ERI_NCLASS; ++c) {
+            const int ncd = ncart_l(eri_class_table[c][2]) * ncart_l(eri_class_table[c][3]);
+            for (int w = 0; w < 2; ++w) {
+                EriClassStore& s = w ? b->store32[c] : b->store64[c];
+                const long long nq = tot[4 * c + w];
+                if (nq > 0x7fffffffLL) return fail("pq_eri_plan: more than 2^31 quartets in one class");
+                s.nq = (int)nq;
+                s.nprimq = tot[4 * c + 2 + w];
+                s.nitems = 0;
+                jobs[c].nq[w] = s.nq;
+                jobs[c].q[w] = nullptr;
+                jobs[c].k[w] = nullptr;
+                if (s.nq) {
+                    const size_t vb = (size_t)s.nq * s.ncomp * (w ? sizeof(float) : sizeof(double));
+                    const size_t jb =
+                        jk_tile_fits(b, c) ? sizeof(double) * (size_t)s.nq * jk_tile_table[c][2] * ncd : 0;
+                    if (s.q.ensure(sizeof(int2) * s.nq) || s.vals.ensure(vb) ||
+                        s.kets.ensure(sizeof(unsigned) * s.nq) || s.jkbuf.ensure(jb))
+                        return -1;
+                    jobs[c].q[w] = (int2*)s.q.p;
+                    jobs[c].k[w] = (unsigned*)s.kets.p;
+                }
+                fb.block0[2 * c + w + 1] = fb.block0[2 * c + w] + (int)((nq + 255) / 256);
+            }
+        }
+        if (upload_jobs(jobs)) return -1;
+        list_fill_batch((const ListJob*)b->d_jobs.p, fb, st);
+    }
+    {
+        std::vector<ListJob> next = b->hjobs.back();   // the next phase's table: a new host copy
+        b->hjobs.push_back(std::move(next));
+    }
+    {
+        std::vector<ListJob>& jobs = b->hjobs.back();
+        ListBlocks ib = {};
+        ib.n = ERI_NCLASS;
+        b->hscans.emplace_back();
+        std::vector<ScanJob>& sc = b->hscans.back();
+        for (int c = 0; c < ERI_NCLASS; ++c) {
+            const bool any = b->store64[c].nq || b->store32[c].nq;
+            jobs[c].nb_items = any ? list_args(b, c).nb : 0;
+            jobs[c].run = item_run(b, c);
+            jobs[c].has32 = b->store32[c].nq > 0;
+            ib.block0[c + 1] = ib.block0[c] + (jobs[c].nb_items + 255) / 256;
+            sc.push_back({jobs[c].b64.nit, jobs[c].b64.its, jobs[c].nb_items, dtot + c});
+        }
+        if (upload_jobs(jobs)) return -1;
+        list_items_count_batch((const ListJob*)b->d_jobs.p, ib, st);
+        if (upload_scans(sc)) return -1;
+        list_scan_batch((const ScanJob*)b->d_scans.p, (int)sc.size(), st);
+    }
+    HIPCHK(hipGetLastError());
+    // every class's item count in one transfer, then the items
     std::vector<long long> nit(ERI_NCLASS, 0);
     HIPCHK(hipMemcpyAsync(nit.data(), dtot, sizeof(long long) * nit.size(), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    for (int c = 0; c < ERI_NCLASS; ++c) {
-        b->store64[c].nitems = b->store32[c].nitems = 0;
-        if ((b->store64[c].nq || b->store32[c].nq) && items_fill(b, c, nit[c], st)) return -1;
+    {
+        std::vector<ListJob> next = b->hjobs.back();   // the next phase's table: a new host copy
+        b->hjobs.push_back(std::move(next));
     }
-    // table of the ket J reductions of every tiled list
+    {
+        std::vector<ListJob>& jobs = b->hjobs.back();
+        ListBlocks tb = {};
+        tb.n = ERI_NCLASS;
+        for (int c = 0; c < ERI_NCLASS; ++c) {
+            EriClassStore& s = b->store64[c];
+            s.nitems = b->store32[c].nitems = 0;
+            jobs[c].nitems = 0;
+            jobs[c].items = nullptr;
+            jobs[c].items32 = nullptr;
+            if (jobs[c].nb_items > 0) {
+                if (nit[c] > 0x7fffffffLL) return fail("J/K work items exceed 2^31");
+                s.nitems = (int)nit[c];
+                const bool f32 = b->store32[c].nq > 0;
+                if (s.items.ensure(sizeof(JkItem) * std::max(1, s.nitems)) ||
+                    (f32 && s.items32.ensure(sizeof(int2) * std::max(1, s.nitems))))
+                    return -1;
+                jobs[c].nitems = s.nitems;
+                jobs[c].items = (JkItem*)s.items.p;
+                jobs[c].items32 = f32 ? (int2*)s.items32.p : nullptr;
+            }
+            tb.block0[c + 1] = tb.block0[c] + (int)((jobs[c].nitems + 255) / 256);
+        }
+        if (upload_jobs(jobs)) return -1;
+        list_items_fill_batch((const ListJob*)b->d_jobs.p, tb, st);
+    }
+    HIPCHK(hipGetLastError());
+    // table of the ket J reductions of every tiled list; every list's tiles
+    // in one buffer and one upload (host copy kept in the handle)
     std::vector<JredList> jr;
+    std::vector<size_t> toff;
+    std::vector<int4>& all = b->hjtiles_all;
+    all.clear();
     long long blocks = 0;
     for (int c = 0; c < ERI_NCLASS; ++c) {
         if (!jk_tile_fits(b, c)) continue;
         const int n3 = ncart_l(eri_class_table[c][2]), n4 = ncart_l(eri_class_table[c][3]);
         for (int w = 0; w < 2; ++w) {
             EriClassStore& s = w ? b->store32[c] : b->store64[c];
+            s.jtiles = nullptr;
+            s.njtile = 0;
             if (!s.nq) continue;
             const int X = eri_class_table[c][5], Y = eri_class_table[c][6];
-            s.hjtiles = jred_tiles(b->hseg[X], b->hseg[Y], b->hsbound[X], b->hsbound[Y], X == Y, b->threshold);
-            const std::vector<int4>& tl = s.hjtiles;   // kept in the handle: no wait for the upload
+            const std::vector<int4> tl =
+                jred_tiles(b->hseg[X], b->hseg[Y], b->hsbound[X], b->hsbound[Y], X == Y, b->threshold);
+            toff.push_back(all.size());
+            all.insert(all.end(), tl.begin(), tl.end());
             s.njtile = (int)tl.size();
-            if (s.jtiles.ensure(sizeof(int4) * std::max<size_t>(1, tl.size()))) return -1;
-            if (!tl.empty())
-                HIPCHK(hipMemcpyAsync(s.jtiles.p, tl.data(), sizeof(int4) * tl.size(), hipMemcpyHostToDevice, st));
             JredList L = jred_list(list_args(b, c), list_bras(s), w, (const double*)s.jkbuf.p, s.nq,
-                                   jk_tile_table[c][2], n3 * n4, n4, (const int4*)s.jtiles.p, s.njtile);
+                                   jk_tile_table[c][2], n3 * n4, n4, nullptr, s.njtile);
             L.block0 = blocks;
             blocks += (long long)L.ntile * L.ncd;
             jr.push_back(L);
+        }
+    }
+    if (b->d_jtiles.ensure(sizeof(int4) * std::max<size_t>(1, all.size()))) return -1;
+    if (!all.empty())
+        HIPCHK(hipMemcpyAsync(b->d_jtiles.p, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice, st));
+    {
+        size_t k = 0;
+        for (int c = 0; c < ERI_NCLASS; ++c) {
+            if (!jk_tile_fits(b, c)) continue;
+            for (int w = 0; w < 2; ++w) {
+                EriClassStore& s = w ? b->store32[c] : b->store64[c];
+                if (!s.nq) continue;
+                s.jtiles = (const int4*)b->d_jtiles.p + toff[k];
+                jr[k].tiles = s.jtiles;
+                ++k;
+            }
         }
     }
     if (blocks > 0x7fffffffLL) return fail("pq_eri_plan: ket J reduction grid too large");
@@ -1285,7 +1425,7 @@ static int jk_launch_class(pq_basis* b, int c, const double* Dj, const double* D
                 EriClassStore& s = w ? s32 : s64;
                 if (!(w ? u32 : u64)) continue;
                 jket_reduce(jred_list(list_args(b, c), list_bras(s), w, (const double*)s.jkbuf.p, s.nq,
-                                      jk_tile_table[c][2], n3 * n4, n4, (const int4*)s.jtiles.p, s.njtile),
+                                      jk_tile_table[c][2], n3 * n4, n4, s.jtiles, s.njtile),
                             dm, J, st);
             }
         }
@@ -1309,7 +1449,9 @@ static int jk_launch_class(pq_basis* b, int c, const double* Dj, const double* D
 // lists in one reduction launch; symmetrise.  With jk_nstreams > 1 the class
 // launches fork from `st` onto side streams (largest stored bytes first, each
 // to the least-loaded stream) and join back before the reduction, so that
-// small high-l classes overlap the large ones.
+// small high-l classes overlap the large ones.  (Each class's reduction
+// launched behind its digestion on the side stream instead was slower: the
+// many small reduction launches queue behind the digestion kernels.)
 static int jk_launch_all(pq_basis* b, const double* Dj, const double* Dk, double* J, double* K, hipStream_t st)
 {
     const long long tot = b->mat_total;
