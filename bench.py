@@ -38,13 +38,22 @@ FP64_PEAK_TFLOPS = 78.6       # MI355X dense FP64 vector
 HBM_PEAK_GBS = 8000.0
 
 
+_ROWS = {}
+
+
 def workload(name):
+    """The workload's Molecule (built from its input rows on every call, as a
+    solve starts) and its config.  The synthetic coordinates themselves are
+    made once per workload: they are the input, not part of the solve."""
     from pychem_amd import geometry, structures
     if name == "water":
         return structures.Molecule(geometry.water(), basis="STO-3G"), {"workload": "H2O/STO-3G RHF", "nbf": 7}
     n = {"c4h10": 4, "c6h14": 6, "c8h18": 8, "c10h22": 10, "c20h42": 20}[name]
     seed = 1000 * n + 10
-    mol = structures.Molecule(geometry.alkane(n, 0.02, seed), basis="6-31G*", cartesian_l=[2])
+    if name not in _ROWS:
+        _ROWS[name] = geometry.alkane(n, 0.02, seed)
+    rows = [list(r) for r in _ROWS[name]]
+    mol = structures.Molecule(rows, basis="6-31G*", cartesian_l=[2])
     return mol, {"workload": "C%dH%d/6-31G* (6d) plain FP64 RHF, conv 1e-10" % (n, 2 * n + 2),
                  "nbf": mol.NOrbitals, "jitter_A": 0.02, "seed": seed}
 

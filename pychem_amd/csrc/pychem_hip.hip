@@ -502,6 +502,20 @@ extern "C" int pq_basis_create_batch(int nmol, int nshell, const int* l, const i
     b->pair_index.assign((size_t)nshell * nshell, -1);
     b->class_pairs.assign(6, {});
     const double PI = 3.14159265358979323846;
+    {
+        long long np = 0, npp = 0;   // upper bounds: every pair of one molecule
+        std::vector<long long> ps(nmol, 0), pn(nmol, 0);
+        for (int s = 0; s < nshell; ++s) {
+            ps[b->shell_mol[s]] += nprim[s];
+            pn[b->shell_mol[s]] += 1;
+        }
+        for (int m = 0; m < nmol; ++m) {
+            np += pn[m] * (pn[m] + 1) / 2;
+            npp += ps[m] * ps[m];
+        }
+        b->pairs.reserve((size_t)np);
+        b->prims.reserve((size_t)npp);
+    }
     for (int sa = 0; sa < nshell; ++sa)
         for (int sb = sa; sb < nshell; ++sb) {
             if (b->shell_mol[sb] != b->shell_mol[sa]) break;   // grouped: later shells are other molecules
@@ -524,7 +538,8 @@ extern "C" int pq_basis_create_batch(int nmol, int nshell, const int* l, const i
                 for (int i2 = 0; i2 < nprim[s2]; ++i2) {
                     const double a1 = exps[prim_off[s1] + i1], a2 = exps[prim_off[s2] + i2];
                     const double p = a1 + a2, sig = 1.0 / p;
-                    const double K = cc[prim_off[s1] + i1] * cc[prim_off[s2] + i2] * pow(PI * sig, 1.5) *
+                    const double q = PI * sig;   // (pi / p)^(3/2) = q sqrt(q)
+                    const double K = cc[prim_off[s1] + i1] * cc[prim_off[s2] + i2] * (q * sqrt(q)) *
                                      exp(-a1 * a2 * sig * ab2);
                     // primitive pairs whose overlap factor underflows contribute nothing
                     if (fabs(K) < 1e-30) continue;

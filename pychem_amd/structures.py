@@ -101,6 +101,21 @@ class ContractedGaussian:
              for lx, ly, lz in cart_components(l)])
 
 
+# Contracted shells depend only on (basis set, element, Cartesian l's), not
+# on the atom's position: atoms of one element share the same (read-only)
+# ContractedGaussian objects.
+_SHELLS = {}
+
+
+def _element_shells(basis_set, label, cartesian_l, lib):
+    key = (basis_set, label, tuple(sorted(set(int(l) for l in cartesian_l))))
+    shells = _SHELLS.get(key)
+    if shells is None:
+        shells = [ContractedGaussian(f, cartesian_l) for f in lib[basis_set][label]]
+        _SHELLS[key] = shells
+    return shells
+
+
 class Atom:
     """Util/structures.py:807-828."""
 
@@ -115,7 +130,7 @@ class Atom:
             raise KeyError("basis set %s not available" % basis_set)
         if self.Label not in lib[basis_set]:
             raise KeyError("element %s not in basis %s" % (self.Label, basis_set))
-        self.Basis = [ContractedGaussian(f, cartesian_l) for f in lib[basis_set][self.Label]]
+        self.Basis = list(_element_shells(basis_set, self.Label, cartesian_l, lib))
         self.NFunctions = sum(c.NAngMom for c in self.Basis)
 
 
