@@ -16,7 +16,12 @@
 // aligned 32-byte run without bounds checks.  X stays exactly symmetric
 // (the start symmetrises F, each step computes only the upper 32 x 32 output
 // tiles and mirrors them), so the right operand's columns are read as rows
-// of X too.  A block's eight waves each compute the whole 32 x 32 tile over a
+// of X too (tiles of 16 x 16 by default, 32 x 32 with PQ_TC2_TILE=32: the
+// smaller tiles give four times the blocks, each with half the operand
+// loads and a quarter of the products; 0.41 against 0.52 ms per
+// purification at m = 194, 0.73 against 0.77 ms at m = 384, where the
+// doubled total operand traffic, 16 ld^3 / edge bytes per step, starts to
+// bind).  A block's eight waves each compute the whole tile over a
 // share of k, straight from global memory with all of a wave's 16-wide k
 // blocks (up to four) of loads in flight at once, and one LDS pass sums the
 // partial tiles.
@@ -24,6 +29,7 @@
 // B[l/16][l%16], result register r of lane l is D[4r + l/16][l%16].  Sizes here are a few hundred rows, so a step is
 // latency-bound: no barrier inside the k loop, one launch per step.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 
 #include <algorithm>
 
@@ -31,7 +37,6 @@
 
 typedef double tc2_d4 __attribute__((ext_vector_type(4)));
 
-#define TC2_T 32   // output tile edge
 
 // Gershgorin interval of row i: lohi[i] = F_ii - r_i, lohi[m + i] = F_ii + r_i,
 // r_i = sum_{j != i} |F_ij| (one wave per row).
@@ -127,9 +132,9 @@ __device__ __forceinline__ double tc2_wave_sum(double v)
 // load; the product is a sum over k in any order, the same order for both
 // operands).  Rows of the left operand: i0 + 16 u + li; columns of the right
 // operand (read as rows of the symmetric X): j0 + 16 v + li.
-template <bool DIRECT, int C>
+template <bool DIRECT, int C, int NU>
 struct Tc2Round {
-    tc2_d4 pa[C][2], pb[C][2], qa[C][2], qb[C][2];
+    tc2_d4 pa[C][NU], pb[C][NU], qa[C][NU], qb[C][NU];
     double keep[C];
     __device__ __forceinline__ void load(const Tc2Step& a, int kb0, int nkb, int i0, int j0, int li, int lq)
     {
@@ -140,7 +145,7 @@ struct Tc2Round {
             const int k = 16 * min(kb, nkb - 1) + 4 * lq;
             keep[c] = (double)(kb < nkb);
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < NU; ++u) {
                 const size_t ia = (size_t)(i0 + 16 * u + li) * ld + k, ib = (size_t)(j0 + 16 * u + li) * ld + k;
                 pa[c][u] = *(const tc2_d4*)(a.P + ia);
                 pb[c][u] = *(const tc2_d4*)(a.P + ib);
@@ -151,22 +156,22 @@ struct Tc2Round {
             }
         }
     }
-    __device__ __forceinline__ void mul(double cp, double cq, tc2_d4 (&acc)[2][2]) const
+    __device__ __forceinline__ void mul(double cp, double cq, tc2_d4 (&acc)[NU][NU]) const
     {
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            tc2_d4 fa[2], fb[2];
+            tc2_d4 fa[NU], fb[NU];
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < NU; ++u) {
                 fa[u] = DIRECT ? keep[c] * pa[c][u] : keep[c] * (cp * pa[c][u] + cq * qa[c][u]);
                 fb[u] = DIRECT ? keep[c] * pb[c][u] : keep[c] * (cp * pb[c][u] + cq * qb[c][u]);
             }
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
-                for (int u = 0; u < 2; ++u)
+                for (int u = 0; u < NU; ++u)
 #pragma unroll
-                    for (int v = 0; v < 2; ++v)
+                    for (int v = 0; v < NU; ++v)
                         acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u][s], fb[v][s], acc[u][v], 0, 0, 0);
         }
     }
@@ -182,9 +187,10 @@ struct Tc2Round {
 // still loads and multiplies (nothing waits on the stop word before the
 // stores: an early exit there made the compiler wait for it before issuing
 // the operand loads).
-template <bool DIRECT, int C>
-__device__ __forceinline__ void tc2_step_body(const Tc2Step& a, double (&red)[TC2_W][4][4][64])
+template <bool DIRECT, int C, int NU>
+__device__ __forceinline__ void tc2_step_body(const Tc2Step& a, double (&red)[TC2_W][NU * NU][4][64])
 {
+    constexpr int TT = 16 * NU;   // output tile edge
     // upper tile (bi, bj), bi <= bj, from the linear block index
     int t = blockIdx.x, bi = 0;
     while (t >= a.nt - bi) {
@@ -192,18 +198,18 @@ __device__ __forceinline__ void tc2_step_body(const Tc2Step& a, double (&red)[TC
         ++bi;
     }
     const int bj = bi + t;
-    const int i0 = bi * TC2_T, j0 = bj * TC2_T;
+    const int i0 = bi * TT, j0 = bj * TT;
     const int ld = a.ld;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lq = lane >> 4;
     // the block's own tile of X_k (and its mirror) is written at the end;
     // its elements are loaded now, with the operands
-    constexpr int NX = TC2_T * TC2_T / (64 * TC2_W);
+    constexpr int NX = (TT * TT + 64 * TC2_W - 1) / (64 * TC2_W);
     double xp[NX], xq[NX];
 #pragma unroll
     for (int n = 0; n < NX; ++n) {
-        const int e = tid + 64 * TC2_W * n;
-        const size_t idx = (size_t)(i0 + (e >> 5)) * ld + j0 + (e & 31);
+        const int e = min(tid + 64 * TC2_W * n, TT * TT - 1);   // past the tile: a repeat, not stored
+        const size_t idx = (size_t)(i0 + e / TT) * ld + j0 + e % TT;
         xp[n] = a.P[idx];
         xq[n] = DIRECT ? 0.0 : a.Q[idx];
     }
@@ -216,7 +222,7 @@ __device__ __forceinline__ void tc2_step_body(const Tc2Step& a, double (&red)[TC
         t2 = a.tin[1];
     }
     const int nkb = ld / 16;
-    Tc2Round<DIRECT, C> rd;
+    Tc2Round<DIRECT, C, NU> rd;
     rd.load(a, wave, nkb, i0, j0, li, lq);
     double cp = 1.0, cq = 0.0;   // X_k = cp P + cq Q
     bool active = true;          // uniform over the grid
@@ -227,29 +233,29 @@ __device__ __forceinline__ void tc2_step_body(const Tc2Step& a, double (&red)[TC
         cp = up ? 0.0 : 2.0;
         cq = up ? 1.0 : -1.0;
     }
-    tc2_d4 acc[2][2];
+    tc2_d4 acc[NU][NU];
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < NU; ++u)
 #pragma unroll
-        for (int v = 0; v < 2; ++v) acc[u][v] = (tc2_d4){0.0, 0.0, 0.0, 0.0};
+        for (int v = 0; v < NU; ++v) acc[u][v] = (tc2_d4){0.0, 0.0, 0.0, 0.0};
     rd.mul(cp, cq, acc);
     for (int kb0 = wave + TC2_W * C; kb0 < nkb; kb0 += TC2_W * C) {   // more rounds (ld > 128 C)
         rd.load(a, kb0, nkb, i0, j0, li, lq);
         rd.mul(cp, cq, acc);
     }
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < NU; ++u)
 #pragma unroll
-        for (int v = 0; v < 2; ++v)
+        for (int v = 0; v < NU; ++v)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) red[wave][2 * u + v][r][lane] = acc[u][v][r];
+            for (int r = 0; r < 4; ++r) red[wave][NU * u + v][r][lane] = acc[u][v][r];
     __syncthreads();
     if (!active) return;
     // output slot e = (sub-tile g, register r, lane l), 1024 of them, summed
     // over the waves; sub-tile g: u = g / 2, v = g % 2
     double trx = 0.0, trx2 = 0.0;
-    for (int e = tid; e < 1024; e += 64 * TC2_W) {
-        const int g = e >> 8, r = (e >> 6) & 3, l = e & 63, u = g >> 1, v = g & 1;
+    for (int e = tid; e < NU * NU * 256; e += 64 * TC2_W) {
+        const int g = e >> 8, r = (e >> 6) & 3, l = e & 63, u = g / NU, v = g % NU;
         double x2 = 0.0;
 #pragma unroll
         for (int w = 0; w < TC2_W; ++w) x2 += red[w][g][r][l];
@@ -261,7 +267,8 @@ __device__ __forceinline__ void tc2_step_body(const Tc2Step& a, double (&red)[TC
 #pragma unroll
     for (int n = 0; n < NX; ++n) {
         const int e = tid + 64 * TC2_W * n;
-        const int gi = i0 + (e >> 5), gj = j0 + (e & 31);
+        if (e >= TT * TT) break;
+        const int gi = i0 + e / TT, gj = j0 + e % TT;
         const double x = DIRECT ? xp[n] : cp * xp[n] + cq * xq[n];
         a.X[(size_t)gi * ld + gj] = x;
         if (bi != bj) a.X[(size_t)gj * ld + gi] = x;
@@ -281,16 +288,16 @@ __device__ __forceinline__ void tc2_step_body(const Tc2Step& a, double (&red)[TC
 // EARLY: read the stop word first and return at once past the recorded last
 // step (a queued step that is most likely not needed then costs only its
 // launch; when it is needed it pays one more memory round trip).
-template <int C, bool EARLY>
+template <int C, bool EARLY, int NU>
 __global__ void __launch_bounds__(64 * TC2_W) tc2_step_kernel(const Tc2Step a)
 {
-    __shared__ double red[TC2_W][4][4][64];   // [wave][sub-tile][register][lane]
+    __shared__ double red[TC2_W][NU * NU][4][64];   // [wave][sub-tile][register][lane]
     if (a.direct) {
-        tc2_step_body<true, C>(a, red);
+        tc2_step_body<true, C, NU>(a, red);
         return;
     }
     if (EARLY && *a.stop < a.k) return;
-    tc2_step_body<false, C>(a, red);
+    tc2_step_body<false, C, NU>(a, red);
 }
 
 // P = sym(X_n) (m x m): n - 1 = the recorded last step, or the last queued
@@ -353,10 +360,38 @@ int tc2_start(const double* F, Tc2Work& w, hipStream_t st)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Output tile edge of the step kernel: 16 (NU = 1) or 32 (NU = 2);
+// PQ_TC2_TILE=32 selects the larger tiles.
+static int tc2_nu()
+{
+    static const int nu = [] {
+        const char* e = getenv("PQ_TC2_TILE");
+        return (e && atoi(e) == 32) ? 2 : 1;
+    }();
+    return nu;
+}
+
+template <int NU>
+static void tc2_launch(int c, bool early, dim3 grid, dim3 block, const Tc2Step& a, hipStream_t st)
+{
+    if (early) {
+        if (c == 1) hipLaunchKernelGGL((tc2_step_kernel<1, true, NU>), grid, block, 0, st, a);
+        else if (c == 2) hipLaunchKernelGGL((tc2_step_kernel<2, true, NU>), grid, block, 0, st, a);
+        else if (c == 3) hipLaunchKernelGGL((tc2_step_kernel<3, true, NU>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((tc2_step_kernel<4, true, NU>), grid, block, 0, st, a);
+    } else {
+        if (c == 1) hipLaunchKernelGGL((tc2_step_kernel<1, false, NU>), grid, block, 0, st, a);
+        else if (c == 2) hipLaunchKernelGGL((tc2_step_kernel<2, false, NU>), grid, block, 0, st, a);
+        else if (c == 3) hipLaunchKernelGGL((tc2_step_kernel<3, false, NU>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((tc2_step_kernel<4, false, NU>), grid, block, 0, st, a);
+    }
+}
+
 int tc2_steps(const Tc2Work& w, int nocc, double tol, int k0, int steps, hipStream_t st, bool early)
 {
     if (k0 < 0 || steps < 0 || k0 + steps > w.max_steps) return -1;
-    const int nt = w.ld / TC2_T;
+    const int nu = tc2_nu();
+    const int nt = w.ld / (16 * nu);   // ld is a multiple of 32
     for (int k = k0; k < k0 + steps; ++k) {
         Tc2Step a;
         a.direct = k == 0;
@@ -374,17 +409,8 @@ int tc2_steps(const Tc2Work& w, int nocc, double tol, int k0, int steps, hipStre
         a.nt = nt;
         const dim3 grid(nt * (nt + 1) / 2), block(64 * TC2_W);
         const int c = tc2_rounds(w.ld);
-        if (early) {
-            if (c == 1) hipLaunchKernelGGL((tc2_step_kernel<1, true>), grid, block, 0, st, a);
-            else if (c == 2) hipLaunchKernelGGL((tc2_step_kernel<2, true>), grid, block, 0, st, a);
-            else if (c == 3) hipLaunchKernelGGL((tc2_step_kernel<3, true>), grid, block, 0, st, a);
-            else hipLaunchKernelGGL((tc2_step_kernel<4, true>), grid, block, 0, st, a);
-        } else {
-            if (c == 1) hipLaunchKernelGGL((tc2_step_kernel<1, false>), grid, block, 0, st, a);
-            else if (c == 2) hipLaunchKernelGGL((tc2_step_kernel<2, false>), grid, block, 0, st, a);
-            else if (c == 3) hipLaunchKernelGGL((tc2_step_kernel<3, false>), grid, block, 0, st, a);
-            else hipLaunchKernelGGL((tc2_step_kernel<4, false>), grid, block, 0, st, a);
-        }
+        if (nu == 1) tc2_launch<1>(c, early, grid, block, a, st);
+        else tc2_launch<2>(c, early, grid, block, a, st);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
