@@ -400,6 +400,20 @@ def main():
                "template <typename real>",
                "__global__ void __launch_bounds__(ERI_BLOCK) eri_k%s(const EriArgs<real> args)" % tag,
                "{ eri_body_%s<real>(args, blockIdx.x * blockDim.x + threadIdx.x); }" % tag]
+        # both lists of the class in one launch (FP64 blocks first): the FP32
+        # list of a mixed-precision plan costs no launch of its own
+        src.append("__global__ void __launch_bounds__(ERI_BLOCK) eri_kmix%s(const EriArgs<double> d, "
+                   "const EriArgs<float> f, int nb64)" % tag)
+        src.append("{")
+        src.append("  if ((int)blockIdx.x < nb64) eri_body_%s<double>(d, blockIdx.x * blockDim.x + threadIdx.x);" % tag)
+        src.append("  else eri_body_%s<float>(f, (blockIdx.x - nb64) * blockDim.x + threadIdx.x);" % tag)
+        src.append("}")
+        src.append("void eri_run_%s_mix(const EriArgs<double>& d, const EriArgs<float>& f, hipStream_t st)" % tag)
+        src.append("{ const int nb64 = (d.nq + ERI_BLOCK - 1) / ERI_BLOCK, nb32 = (f.nq + ERI_BLOCK - 1) / ERI_BLOCK;")
+        src.append("  if (nb64 + nb32 > 0) hipLaunchKernelGGL(eri_kmix%s, dim3(nb64 + nb32), dim3(ERI_BLOCK), 0, st, d, f, nb64); }" % tag)
+        src.append("size_t eri_scratch_%s_mix()" % tag)
+        src.append("{ hipFuncAttributes f; return hipFuncGetAttributes(&f, (const void*)eri_kmix%s) == hipSuccess ? "
+                   "f.localSizeBytes : (size_t)1 << 30; }" % tag)
         jsrc = ["// generated by gen_eri.py -- do not edit", '#include "../jk_common.h"']
         for rn, rt in (("d", "double"), ("f", "float")):
             src.append("void eri_run_%s_%s(const EriArgs<%s>& a, hipStream_t st)" % (tag, rn, rt))
@@ -424,9 +438,8 @@ def main():
     fz = ["// generated by gen_eri.py -- do not edit", '#include "../eri_common.h"']
     fz += ['#include "body_%s.h"' % t for t in FUSED]
     fz.append("template <typename real>")
-    fz.append("__global__ void __launch_bounds__(ERI_BLOCK) eri_fused_k(const EriFusedArgs<real> a)")
+    fz.append("__device__ __forceinline__ void eri_fused_block(const EriFusedArgs<real>& a, int b)")
     fz.append("{")
-    fz.append("  const int b = blockIdx.x;")
     fz.append("  int j = 0;")
     fz.append("  while (j + 1 < a.n && b >= a.block0[j + 1]) ++j;")
     fz.append("  const int q = (b - a.block0[j]) * ERI_BLOCK + threadIdx.x;")
@@ -436,6 +449,19 @@ def main():
     fz.append("  default: break;")
     fz.append("  }")
     fz.append("}")
+    fz.append("template <typename real>")
+    fz.append("__global__ void __launch_bounds__(ERI_BLOCK) eri_fused_k(const EriFusedArgs<real> a)")
+    fz.append("{ eri_fused_block<real>(a, blockIdx.x); }")
+    fz.append("__global__ void __launch_bounds__(ERI_BLOCK) eri_fused_mix_k(const EriFusedArgs<double> d, "
+              "const EriFusedArgs<float> f)")
+    fz.append("{")
+    fz.append("  const int nd = d.n > 0 ? d.block0[d.n] : 0;")
+    fz.append("  if ((int)blockIdx.x < nd) eri_fused_block<double>(d, blockIdx.x);")
+    fz.append("  else eri_fused_block<float>(f, blockIdx.x - nd);")
+    fz.append("}")
+    fz.append("void eri_fused_run_mix(const EriFusedArgs<double>& d, const EriFusedArgs<float>& f, hipStream_t st)")
+    fz.append("{ const int nb = (d.n > 0 ? d.block0[d.n] : 0) + (f.n > 0 ? f.block0[f.n] : 0);")
+    fz.append("  if (nb > 0) hipLaunchKernelGGL(eri_fused_mix_k, dim3(nb), dim3(ERI_BLOCK), 0, st, d, f); }")
     for rn, rt in (("d", "double"), ("f", "float")):
         fz.append("void eri_fused_run_%s(const EriFusedArgs<%s>& a, hipStream_t st)" % (rn, rt))
         fz.append("{ if (a.n > 0) hipLaunchKernelGGL(eri_fused_k<%s>, dim3(a.block0[a.n]), dim3(ERI_BLOCK), 0, st, a); }" % rt)
@@ -470,6 +496,8 @@ def main():
             d.append("size_t eri_scratch_%s_%s();" % (tag, rn))
             d.append("void jk_run_%s_%s(const JkArgs<%s>&, hipStream_t);" % (tag, rn, rt))
             d.append("void scatter_run_%s_%s(const ScatterArgs<%s>&, hipStream_t);" % (tag, rn, rt))
+        d.append("void eri_run_%s_mix(const EriArgs<double>&, const EriArgs<float>&, hipStream_t);" % tag)
+        d.append("size_t eri_scratch_%s_mix();" % tag)
     d.append("void onee_fused_run(const OneeFusedArgs& a, hipStream_t st);")
     d.append("// classes of the fused heavy-class launch: slot in eri_fused_k, or -1")
     d.append("#define ERI_NFUSED %d" % len(FUSED))
@@ -479,13 +507,15 @@ def main():
     d.append("};")
     d.append("void eri_fused_run_d(const EriFusedArgs<double>&, hipStream_t);")
     d.append("void eri_fused_run_f(const EriFusedArgs<float>&, hipStream_t);")
+    d.append("// both precisions' heavy classes in one launch (FP64 blocks first)")
+    d.append("void eri_fused_run_mix(const EriFusedArgs<double>&, const EriFusedArgs<float>&, hipStream_t);")
     d.append("// la, lb, lc, ld, components, bra pair class, ket pair class")
     d.append("static const int eri_class_table[ERI_NCLASS][7] = {")
     for la, lb, lc, ld, ncomp, X, Y, nbody in table:
         d.append("  {%d, %d, %d, %d, %d, %d, %d}," % (la, lb, lc, ld, ncomp, X, Y))
     d.append("};")
     d.append("// per-lane scratch (spill) bytes of each class kernel, FP64 and FP32 forms")
-    for rn in ("d", "f"):
+    for rn in ("d", "f", "mix"):
         d.append("static size_t (*const eri_scratch_table_%s[ERI_NCLASS])() = {" % rn)
         d.append("  " + ", ".join("eri_scratch_%d%d%d%d_%s" % (t[0], t[1], t[2], t[3], rn) for t in table))
         d.append("};")
@@ -502,6 +532,10 @@ def main():
     d.append("};")
     d.append("static inline void jktile_launch(int cls, const JkTileArgs& a, int block, hipStream_t st)")
     d.append("{ jktile_table[cls](a, block, st); }")
+    d.append("// both lists of a class in one launch")
+    d.append("static void (*const eri_table_mix[ERI_NCLASS])(const EriArgs<double>&, const EriArgs<float>&, hipStream_t) = {")
+    d.append("  " + ", ".join("eri_run_%d%d%d%d_mix" % (t[0], t[1], t[2], t[3]) for t in table))
+    d.append("};")
     for kname, targ in (("eri", "EriArgs"), ("jk", "JkArgs"), ("scatter", "ScatterArgs")):
         for rn, rt in (("d", "double"), ("f", "float")):
             d.append("static void (*const %s_table_%s[ERI_NCLASS])(const %s<%s>&, hipStream_t) = {" % (kname, rn, targ, rt))

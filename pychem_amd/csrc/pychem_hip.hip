@@ -1213,23 +1213,29 @@ extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold,
 }
 
 // All class kernels over the planned lists (the ERI-generation pass).
-// The class kernels fork from `st` onto the side streams (costliest first,
-// each to the least-loaded stream; cost ~ primitive quartets x components) and
-// join back, so that the classes overlap.  Kernels that spill (per-lane
-// scratch: the runtime sizes scratch per hardware queue for a full device of
-// waves) stay on `st`, the one queue that always carried them.
+// Each class is one launch: its FP64 and FP32 lists (a mixed-precision plan)
+// together, the FP64 blocks first.  The launches fork from `st` onto the
+// side streams (costliest first, each to the least-loaded stream; cost ~
+// primitive quartets x components) and join back, so that the classes
+// overlap.  Kernels that spill (per-lane scratch: the runtime sizes scratch
+// per hardware queue for a full device of waves) stay on `st`, the one queue
+// that always carried them; the heavy classes whose kernels spill run there
+// as one fused launch, both precisions together.
 extern "C" int pq_eri_compute(pq_basis* b, void* stream)
 {
     if (!b) return fail("pq_eri_compute: null basis");
     hipStream_t st = (hipStream_t)stream;
     if (b->timing && ensure_events(b)) return -1;
     std::vector<std::pair<double, int>> order;
-    for (int c = 0; c < ERI_NCLASS; ++c)
+    for (int c = 0; c < ERI_NCLASS; ++c) {
+        double cost = 0.0;
         for (int w = 0; w < 2; ++w) {
             EriClassStore& s = w ? b->store32[c] : b->store64[c];
             b->ev_cls_used[2 * c + w] = 0;
-            if (s.nq) order.push_back({(double)s.nprimq * s.ncomp + (double)s.nq * s.ncomp, 2 * c + w});
+            if (s.nq) cost += (double)s.nprimq * s.ncomp + (double)s.nq * s.ncomp;
         }
+        if (cost > 0.0) order.push_back({cost, c});
+    }
     std::sort(order.begin(), order.end(), [](const std::pair<double, int>& u, const std::pair<double, int>& v) {
         return u.first > v.first;
     });
@@ -1240,62 +1246,81 @@ extern "C" int pq_eri_compute(pq_basis* b, void* stream)
         HIPCHK(hipEventRecord(b->ev_fork, st));
         for (int k = 0; k < ns; ++k) HIPCHK(hipStreamWaitEvent(b->jk_side[k], b->ev_fork, 0));
     }
-    // the heavy classes (their kernels use scratch) in one launch per
-    // precision on the caller's stream, so that they run side by side while
-    // scratch stays on one queue; the rest below
-    for (int w = 0; w < 2; ++w) {
+    auto args_d = [&](int c) {
+        EriClassStore& s = b->store64[c];
+        return EriArgs<double>{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const PrimPair<double>*)b->d_prims.p,
+                               (double*)s.vals.p, s.nq};
+    };
+    auto args_f = [&](int c) {
+        EriClassStore& s = b->store32[c];
+        return EriArgs<float>{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p, (const PrimPair<double>*)b->d_prims.p,
+                              (float*)s.vals.p, s.nq};
+    };
+    auto mark = [&](int c, int k, hipStream_t sk) -> int {   // timing events of both lists of class c
+        if (!b->timing) return 0;
+        for (int w = 0; w < 2; ++w)
+            if ((w ? b->store32[c] : b->store64[c]).nq) {
+                HIPCHK(hipEventRecord(b->ev_cls[2 * c + w][k], sk));
+                if (k) b->ev_cls_used[2 * c + w] = 1;
+            }
+        return 0;
+    };
+    // the heavy classes (their kernels use scratch): one launch on the
+    // caller's stream, both precisions, the block index selects the class
+    {
         EriFusedArgs<double> fd = {};
         EriFusedArgs<float> ff = {};
-        int nf = 0;
-        fd.block0[0] = ff.block0[0] = 0;
         for (int c = 0; c < ERI_NCLASS; ++c) {
-            EriClassStore& s = w ? b->store32[c] : b->store64[c];
-            if (eri_fused_slot[c] < 0 || !s.nq) continue;
-            EriArgs<double> ad{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p,
-                               (const PrimPair<double>*)b->d_prims.p, (double*)s.vals.p, s.nq};
-            EriArgs<float> af{(const int2*)s.q.p, (const PairHdr*)b->d_pairs.p,
-                              (const PrimPair<double>*)b->d_prims.p, (float*)s.vals.p, s.nq};
-            fd.cls[nf] = ad;
-            ff.cls[nf] = af;
-            fd.which[nf] = ff.which[nf] = eri_fused_slot[c];
-            fd.block0[nf + 1] = ff.block0[nf + 1] = fd.block0[nf] + (s.nq + ERI_BLOCK - 1) / ERI_BLOCK;
-            ++nf;
+            if (eri_fused_slot[c] < 0) continue;
+            if (b->store64[c].nq) {
+                fd.cls[fd.n] = args_d(c);
+                fd.which[fd.n] = eri_fused_slot[c];
+                fd.block0[fd.n + 1] = fd.block0[fd.n] + (b->store64[c].nq + ERI_BLOCK - 1) / ERI_BLOCK;
+                ++fd.n;
+            }
+            if (b->store32[c].nq) {
+                ff.cls[ff.n] = args_f(c);
+                ff.which[ff.n] = eri_fused_slot[c];
+                ff.block0[ff.n + 1] = ff.block0[ff.n] + (b->store32[c].nq + ERI_BLOCK - 1) / ERI_BLOCK;
+                ++ff.n;
+            }
         }
-        fd.n = ff.n = nf;
-        if (!nf) continue;
-        if (b->timing)
+        if (fd.n || ff.n) {
             for (int c = 0; c < ERI_NCLASS; ++c)
-                if (eri_fused_slot[c] >= 0 && (w ? b->store32[c] : b->store64[c]).nq)
-                    HIPCHK(hipEventRecord(b->ev_cls[2 * c + w][0], st));
-        if (w) eri_fused_run_f(ff, st);
-        else eri_fused_run_d(fd, st);
-        HIPCHK(hipGetLastError());
-        if (b->timing)
+                if (eri_fused_slot[c] >= 0 && mark(c, 0, st)) return -1;
+            if (ff.n) eri_fused_run_mix(fd, ff, st);
+            else eri_fused_run_d(fd, st);
+            HIPCHK(hipGetLastError());
             for (int c = 0; c < ERI_NCLASS; ++c)
-                if (eri_fused_slot[c] >= 0 && (w ? b->store32[c] : b->store64[c]).nq) {
-                    HIPCHK(hipEventRecord(b->ev_cls[2 * c + w][1], st));
-                    b->ev_cls_used[2 * c + w] = 1;
-                }
+                if (eri_fused_slot[c] >= 0 && mark(c, 1, st)) return -1;
+        }
     }
     double load[4] = {0.0, 0.0, 0.0, 0.0};
     for (const auto& o : order) {
+        const int c = o.second;
+        if (eri_fused_slot[c] >= 0) continue;
         int k = 0;
         for (int t = 1; t < ns; ++t)
             if (load[t] < load[k]) k = t;
-        const int c = o.second / 2, w = o.second % 2, slot = o.second;
-        if (eri_fused_slot[c] >= 0) continue;
-        const bool spills = (w ? eri_scratch_table_f[c] : eri_scratch_table_d[c])() > 0;
+        const bool u64 = b->store64[c].nq > 0, u32 = b->store32[c].nq > 0;
+        const bool spills = (u64 && u32) ? eri_scratch_table_mix[c]() > 0
+                                         : (u64 ? eri_scratch_table_d[c]() > 0 : eri_scratch_table_f[c]() > 0);
         if (!spills) load[k] += o.first;
         hipStream_t sk = (ns > 1 && !spills) ? b->jk_side[k] : st;
-        EriClassStore& s = w ? b->store32[c] : b->store64[c];
-        if (b->timing) HIPCHK(hipEventRecord(b->ev_cls[slot][0], sk));
-        int rc = w ? run_class<float>(b, c, (const int2*)s.q.p, s.nq, (float*)s.vals.p, sk)
-                   : run_class<double>(b, c, (const int2*)s.q.p, s.nq, (double*)s.vals.p, sk);
-        if (rc) return -1;
-        if (b->timing) {
-            HIPCHK(hipEventRecord(b->ev_cls[slot][1], sk));
-            b->ev_cls_used[slot] = 1;
+        if (mark(c, 0, sk)) return -1;
+        if (u64 && u32) {
+            eri_table_mix[c](args_d(c), args_f(c), sk);
+            HIPCHK(hipGetLastError());
+        } else if (u64) {
+            if (run_class<double>(b, c, (const int2*)b->store64[c].q.p, b->store64[c].nq, (double*)b->store64[c].vals.p,
+                                  sk))
+                return -1;
+        } else {
+            if (run_class<float>(b, c, (const int2*)b->store32[c].q.p, b->store32[c].nq, (float*)b->store32[c].vals.p,
+                                 sk))
+                return -1;
         }
+        if (mark(c, 1, sk)) return -1;
     }
     if (ns > 1)
         for (int k = 0; k < ns; ++k) {
