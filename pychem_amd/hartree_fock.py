@@ -127,8 +127,11 @@ def make_core_matrices(molecule, wait=True):
     global _ORTHO_POOL
     eng = get_engine(molecule)
     S, T, V = eng.one_electron()
+    H = T + V
     molecule.Overlap = S.to(HOST)
-    molecule.Core = (T + V).to(HOST)
+    molecule.Core = H.to(HOST)
+    # the device copies serve the device SCF loop (no second upload)
+    molecule._device_core = (S, H)
 
     def ortho():
         s, U = torch.linalg.eigh(molecule.Overlap)
@@ -558,8 +561,12 @@ def _scf_device(molecule, state, conv, maxiter, use_diis, diis_start, verbose, t
     matrix."""
     eng = get_engine(molecule)
     dev = eng.device
-    H = molecule.Core.to(dev)
-    S = molecule.Overlap.to(dev)
+    dc = getattr(molecule, "_device_core", None)
+    if dc is not None and dc[0].device == dev:
+        S, H = dc
+    else:
+        H = molecule.Core.to(dev)
+        S = molecule.Overlap.to(dev)
     X = molecule.X.to(dev)
     Xt = X.T.contiguous()
     na, nb = state.NAlpha, state.NBeta

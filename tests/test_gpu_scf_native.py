@@ -125,3 +125,20 @@ def test_native_loop_purification_continuation(gpu, monkeypatch, case):
     # long (UHF) iteration may then take a few cycles more or less
     assert abs(r1["iterations"] - r2["iterations"]) <= max(1, r1["iterations"] // 6)
     assert np.allclose(r1["history"][:3], r2["history"][:3], rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("rows,mult", [([["H", 1.0, 0, 0, 0]], 2), ([["He", 2.0, 0, 0, 0]], 1)],
+                         ids=["H_atom_doublet", "He_atom"])
+def test_native_loop_one_function_atoms(gpu, monkeypatch, rows, mult):
+    """One basis function (STO-3G H, He): every spin is empty or full (the
+    projector is 0 or 1 without purification), GEMMs of size 1; both loops
+    and the oracle agree."""
+    from pychem_amd import hartree_fock
+
+    def make():
+        return structures.Molecule(rows, basis="STO-3G", multiplicity=mult)
+    m1, r1, m2, r2 = _both(monkeypatch, make, "CORE", conv=1e-10, maxiter=50)
+    assert r1["converged"] and r2["converged"]
+    assert abs(r1["energy"] - r2["energy"]) < 1e-12
+    o = oscf.run_rhf(make(), conv=1e-10, maxiter=50, diis=oscf.PULAY)
+    assert abs(r1["energy"] - o["energy"]) < 1e-10
