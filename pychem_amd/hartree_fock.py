@@ -217,6 +217,17 @@ def make_MOs(molecule, state):
         state.Beta.Energies, state.Beta.MOs = state.Alpha.Energies, state.Alpha.MOs
 
 
+def _make_MOs_device(molecule, state, X, Xt, Fa, Fb):
+    """make_MOs for device Fock matrices: Xt F X and X C on the device, the
+    diagonalisation on the host (one per spin, once when the spins share F)."""
+    spins = ((state.Alpha, Fa),) if Fb is Fa else ((state.Alpha, Fa), (state.Beta, Fb))
+    for m, F in spins:
+        e, U = torch.linalg.eigh((Xt @ F @ X).to(HOST))
+        m.Energies, m.MOs = e, (X @ U.to(X.device)).to(HOST)
+    if len(spins) == 1:
+        state.Beta.Energies, state.Beta.MOs = state.Alpha.Energies, state.Alpha.MOs
+
+
 def make_density_matrices(molecule, state):
     """D^sigma = C_occ C_occ^T (Methods/hartree_fock.py:179-184)."""
     same = state.Alpha.MOs is state.Beta.MOs and state.NAlpha == state.NBeta
@@ -674,7 +685,10 @@ def _scf_device(molecule, state, conv, maxiter, use_diis, diis_start, verbose, t
         state.Beta.Fock = state.Alpha.Fock if Fb is Fa else Fb.to(HOST)
     state.Alpha.Density = Da.to(HOST)
     state.Beta.Density = state.Alpha.Density if Db is Da else Db.to(HOST)
-    make_MOs(molecule, state)
+    if Fa is not None and isinstance(Fa, torch.Tensor) and Fa.device.type == "cuda":
+        _make_MOs_device(molecule, state, X, Xt, Fa, Fb)
+    else:
+        make_MOs(molecule, state)
     state.Alpha.sort_occupied()
     if state.Beta.MOs is state.Alpha.MOs:
         state.Beta.Energies = state.Alpha.Energies
