@@ -3,8 +3,8 @@
 #include <hip/hip_runtime.h>
 
 // Work layout of one purification (4 ld^2 + 2 max_steps + 3 doubles, ld = m
-// rounded up to 32): two (X, X^2) pairs of ld x ld matrices (zero outside
-// the m x m block) that alternate between steps, the Gershgorin bounds, per
+// rounded up to 32): two pairs of ld x ld matrices (zero outside the m x m
+// block; (X^2, 2 X - X^2) or (X, X^2)) that alternate between steps, the Gershgorin bounds, per
 // step k the traces (tr X_k, tr X_k^2), and the stop word (index of the last
 // step).
 inline int tc2_ld(int m) { return (m + 31) / 32 * 32; }
@@ -15,6 +15,10 @@ struct Tc2Work {
     int* stop;          // one double's room, right after the traces
     double* x0;         // start matrix (the X slot of pair 1 until step 1 overwrites it)
     int m, ld, max_steps;
+    // candidate form (default; PQ_TC2_CAND=0 selects the X, X^2 pairs):
+    // pair[p] holds X^2 at +0 and 2 X - X^2 at +ld*ld, and a step loads only
+    // the one its input traces choose
+    int cand;
 };
 inline long long tc2_work_doubles(int m, int max_steps)
 {
@@ -27,8 +31,9 @@ Tc2Work tc2_work(double* work, int m, int max_steps);
 // (F symmetrised).
 int tc2_start(const double* F, Tc2Work& w, hipStream_t st);
 // Steps k0 .. k0 + steps - 1 (one launch each): step k forms X_k from the
-// pair and traces of step k - 1 (X0 for k = 0) and writes X_k, X_k^2 and
-// their traces into slot k.  The step after the one whose input has
+// pair and traces of step k - 1 (X0 for k = 0) and writes its pair (X_k^2
+// and 2 X_k - X_k^2, or X_k and X_k^2) and the traces of X_k and X_k^2 into
+// slot k.  The step after the one whose input has
 // tr(X - X^2) < tol is the last; it records its index in the stop word and
 // later queued steps do nothing (early: such steps return at once; otherwise
 // they still load and multiply, and a needed step saves a memory round trip).

@@ -284,20 +284,119 @@ __device__ __forceinline__ void tc2_step_body(const Tc2Step& a, double (&red)[TC
     }
 }
 
+// Candidate form (the default; the step is bound by its operand loads at
+// every size measured, m = 194 and 384): step k stores both candidates for X_{k+1}, U_k = X_k^2 (in the X
+// slot) and W_k = 2 X_k - X_k^2 (in the X^2 slot), with the traces of X_k
+// and X_k^2.  Step k + 1 reads the traces first, chooses U or W and loads
+// only that matrix: half the operand bytes of loading X and X^2 together,
+// for one more memory round trip.  The block's own tile of X_k is read in
+// the layout of the products, so that W = 2 X - X^2 is formed where X^2 is.
+template <bool DIRECT, int C, int NU>
+__device__ __forceinline__ void tc2_cand_body(const Tc2Step& a, double (&red)[TC2_W][NU * NU][4][64])
+{
+    int t = blockIdx.x, bi = 0;
+    while (t >= a.nt - bi) {
+        t -= a.nt - bi;
+        ++bi;
+    }
+    const int bj = bi + t;
+    const int i0 = bi * 16 * NU, j0 = bj * 16 * NU;
+    const int ld = a.ld;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lq = lane >> 4;
+    const double* Xk = a.P;   // X_k: x0 at step 0, else the chosen candidate
+    bool active = true;       // uniform over the grid
+    if (!DIRECT) {
+        const int stop = *a.stop;
+        const double t1 = a.tin[0], t2 = a.tin[1];
+        active = !(stop < a.k);
+        if (active && t1 - t2 < a.tol && blockIdx.x == 0 && threadIdx.x == 0) *a.stop = a.k;
+        const bool up = fabs(t2 - a.nocc) < fabs(2.0 * t1 - t2 - a.nocc);
+        Xk = up ? a.P : a.Q;
+    }
+    constexpr int NOUT = NU * NU * 256;
+    constexpr int NO = (NOUT + 64 * TC2_W - 1) / (64 * TC2_W);
+    auto slot = [&](int e, int& gi, int& gj) {   // output slot e = (sub-tile g, register r, lane l)
+        const int g = e >> 8, r = (e >> 6) & 3, l = e & 63, u = g / NU, v = g % NU;
+        gi = i0 + 16 * u + 4 * r + (l >> 4);
+        gj = j0 + 16 * v + (l & 15);
+    };
+    double xo[NO];
+#pragma unroll
+    for (int n = 0; n < NO; ++n) {
+        int gi, gj;
+        slot(min(tid + 64 * TC2_W * n, NOUT - 1), gi, gj);   // past the tile: a repeat, not stored
+        xo[n] = Xk[(size_t)gi * ld + gj];
+    }
+    Tc2Step s = a;
+    s.P = Xk;
+    const int nkb = ld / 16;
+    tc2_d4 acc[NU][NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u)
+#pragma unroll
+        for (int v = 0; v < NU; ++v) acc[u][v] = (tc2_d4){0.0, 0.0, 0.0, 0.0};
+    Tc2Round<true, C, NU> rd;
+    for (int kb0 = wave; kb0 < nkb; kb0 += TC2_W * C) {
+        rd.load(s, kb0, nkb, i0, j0, li, lq);
+        rd.mul(1.0, 0.0, acc);
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u)
+#pragma unroll
+        for (int v = 0; v < NU; ++v)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) red[wave][NU * u + v][r][lane] = acc[u][v][r];
+    __syncthreads();
+    if (!active) return;
+    double trx = 0.0, trx2 = 0.0;
+#pragma unroll
+    for (int n = 0; n < NO; ++n) {
+        const int e = tid + 64 * TC2_W * n;
+        if (e >= NOUT) break;
+        const int g = e >> 8, r = (e >> 6) & 3, l = e & 63;
+        double x2 = 0.0;
+#pragma unroll
+        for (int w = 0; w < TC2_W; ++w) x2 += red[w][g][r][l];
+        int gi, gj;
+        slot(e, gi, gj);
+        const double x = xo[n], wv = 2.0 * x - x2;
+        a.X[(size_t)gi * ld + gj] = x2;
+        a.X2[(size_t)gi * ld + gj] = wv;
+        if (bi != bj) {
+            a.X[(size_t)gj * ld + gi] = x2;
+            a.X2[(size_t)gj * ld + gi] = wv;
+        } else if (gi == gj) {
+            trx += x;
+            trx2 += x2;
+        }
+    }
+    if (bi == bj) {
+        trx = tc2_wave_sum(trx);
+        trx2 = tc2_wave_sum(trx2);
+        if (lane == 0) {
+            atomicAdd(&a.tout[0], trx);
+            atomicAdd(&a.tout[1], trx2);
+        }
+    }
+}
+
 // C = k blocks of 16 per wave and round: ceil(ld / (16 TC2_W)), at most 4.
 // EARLY: read the stop word first and return at once past the recorded last
 // step (a queued step that is most likely not needed then costs only its
 // launch; when it is needed it pays one more memory round trip).
-template <int C, bool EARLY, int NU>
+template <int C, bool EARLY, int NU, bool CAND>
 __global__ void __launch_bounds__(64 * TC2_W) tc2_step_kernel(const Tc2Step a)
 {
     __shared__ double red[TC2_W][NU * NU][4][64];   // [wave][sub-tile][register][lane]
     if (a.direct) {
-        tc2_step_body<true, C, NU>(a, red);
+        if (CAND) tc2_cand_body<true, C, NU>(a, red);
+        else tc2_step_body<true, C, NU>(a, red);
         return;
     }
     if (EARLY && *a.stop < a.k) return;
-    tc2_step_body<false, C, NU>(a, red);
+    if (CAND) tc2_cand_body<false, C, NU>(a, red);
+    else tc2_step_body<false, C, NU>(a, red);
 }
 
 // P = sym(X_n) (m x m): n - 1 = the recorded last step, or the last queued
@@ -305,7 +404,7 @@ __global__ void __launch_bounds__(64 * TC2_W) tc2_step_kernel(const Tc2Step a)
 // traces.
 __global__ void tc2_finish_kernel(const double* __restrict__ pair0, const double* __restrict__ pair1,
                                   const double* __restrict__ tslots, const int* __restrict__ stop, int queued,
-                                  double nocc, int m, int ld, double* P)
+                                  double nocc, int m, int ld, int cand, double* P)
 {
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= (long long)m * m) return;
@@ -316,8 +415,9 @@ __global__ void tc2_finish_kernel(const double* __restrict__ pair0, const double
     const double t1 = tslots[2 * last], t2 = tslots[2 * last + 1];
     const bool up = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc);
     const size_t ij = (size_t)i * ld + j, ji = (size_t)j * ld + i;
-    const double a = up ? X2p[ij] : 2.0 * Xp[ij] - X2p[ij];
-    const double b = up ? X2p[ji] : 2.0 * Xp[ji] - X2p[ji];
+    // candidate form: the pair holds U = X^2 and W = 2 X - X^2 themselves
+    const double a = cand ? (up ? Xp[ij] : X2p[ij]) : (up ? X2p[ij] : 2.0 * Xp[ij] - X2p[ij]);
+    const double b = cand ? (up ? Xp[ji] : X2p[ji]) : (up ? X2p[ji] : 2.0 * Xp[ji] - X2p[ji]);
     P[t] = 0.5 * (a + b);
 }
 
@@ -341,6 +441,11 @@ Tc2Work tc2_work(double* work, int m, int max_steps)
     w.t = w.bounds + 2;
     w.stop = (int*)(w.t + 2 * max_steps);
     w.x0 = nullptr;
+    static const int cand_env = [] {
+        const char* e = getenv("PQ_TC2_CAND");
+        return e ? atoi(e) : -1;
+    }();
+    w.cand = cand_env != 0;
     return w;
 }
 
@@ -371,19 +476,19 @@ static int tc2_nu()
     return nu;
 }
 
-template <int NU>
+template <int NU, bool CAND>
 static void tc2_launch(int c, bool early, dim3 grid, dim3 block, const Tc2Step& a, hipStream_t st)
 {
     if (early) {
-        if (c == 1) hipLaunchKernelGGL((tc2_step_kernel<1, true, NU>), grid, block, 0, st, a);
-        else if (c == 2) hipLaunchKernelGGL((tc2_step_kernel<2, true, NU>), grid, block, 0, st, a);
-        else if (c == 3) hipLaunchKernelGGL((tc2_step_kernel<3, true, NU>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((tc2_step_kernel<4, true, NU>), grid, block, 0, st, a);
+        if (c == 1) hipLaunchKernelGGL((tc2_step_kernel<1, true, NU, CAND>), grid, block, 0, st, a);
+        else if (c == 2) hipLaunchKernelGGL((tc2_step_kernel<2, true, NU, CAND>), grid, block, 0, st, a);
+        else if (c == 3) hipLaunchKernelGGL((tc2_step_kernel<3, true, NU, CAND>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((tc2_step_kernel<4, true, NU, CAND>), grid, block, 0, st, a);
     } else {
-        if (c == 1) hipLaunchKernelGGL((tc2_step_kernel<1, false, NU>), grid, block, 0, st, a);
-        else if (c == 2) hipLaunchKernelGGL((tc2_step_kernel<2, false, NU>), grid, block, 0, st, a);
-        else if (c == 3) hipLaunchKernelGGL((tc2_step_kernel<3, false, NU>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((tc2_step_kernel<4, false, NU>), grid, block, 0, st, a);
+        if (c == 1) hipLaunchKernelGGL((tc2_step_kernel<1, false, NU, CAND>), grid, block, 0, st, a);
+        else if (c == 2) hipLaunchKernelGGL((tc2_step_kernel<2, false, NU, CAND>), grid, block, 0, st, a);
+        else if (c == 3) hipLaunchKernelGGL((tc2_step_kernel<3, false, NU, CAND>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((tc2_step_kernel<4, false, NU, CAND>), grid, block, 0, st, a);
     }
 }
 
@@ -409,8 +514,13 @@ int tc2_steps(const Tc2Work& w, int nocc, double tol, int k0, int steps, hipStre
         a.nt = nt;
         const dim3 grid(nt * (nt + 1) / 2), block(64 * TC2_W);
         const int c = tc2_rounds(w.ld);
-        if (nu == 1) tc2_launch<1>(c, early, grid, block, a, st);
-        else tc2_launch<2>(c, early, grid, block, a, st);
+        if (w.cand) {
+            if (nu == 1) tc2_launch<1, true>(c, early, grid, block, a, st);
+            else tc2_launch<2, true>(c, early, grid, block, a, st);
+        } else {
+            if (nu == 1) tc2_launch<1, false>(c, early, grid, block, a, st);
+            else tc2_launch<2, false>(c, early, grid, block, a, st);
+        }
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -421,7 +531,7 @@ int tc2_finish(const Tc2Work& w, int nocc, int queued, double* P, hipStream_t st
     if (queued < 1 || queued > w.max_steps) return -1;
     hipLaunchKernelGGL(tc2_finish_kernel, dim3(nblk((long long)m * m)), dim3(256), 0, st,
                        (const double*)w.pair[0], (const double*)w.pair[1], (const double*)w.t, (const int*)w.stop,
-                       queued, (double)nocc, m, w.ld, P);
+                       queued, (double)nocc, m, w.ld, w.cand, P);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
