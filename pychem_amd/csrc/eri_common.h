@@ -38,6 +38,7 @@ struct EriArgs {
     const PrimPair<double>* prims;   // FP64 for both precisions
     real* out;
     int nq;
+    const int* order;   // thread t computes quartet order[t] (NULL: quartet t)
 };
 
 // Several classes in one launch: blocks [block0[j], block0[j + 1]) run

@@ -174,10 +174,11 @@ def gen_eri_class(la, lb, lc, ld):
 
     lines = []
     lines.append("template <typename real>")
-    lines.append("__device__ __forceinline__ void eri_body_%s(const EriArgs<real>& args, const int q)" % tag)
+    lines.append("__device__ __forceinline__ void eri_body_%s(const EriArgs<real>& args, const int t)" % tag)
     lines.append("{")
     lines.append("  const int nq = args.nq;")
-    lines.append("  if (q >= nq) return;")
+    lines.append("  if (t >= nq) return;")
+    lines.append("  const int q = args.order ? (int)min((unsigned)args.order[t], (unsigned)(nq - 1)) : t;")
     lines.append("  const int2 pr = args.quartets[q];")
     lines.append("  const PairHdr bh = args.pairs[pr.x];")
     lines.append("  const PairHdr kh = args.pairs[pr.y];")

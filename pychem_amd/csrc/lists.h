@@ -68,6 +68,7 @@ struct ListJob {
     long long run;         // work-item length
     int nb_items;          // bras whose items are counted (0: class without quartets)
     int has32;             // the class has an FP32 list
+    int* ord[2];           // quartet order of the ERI kernels (list_order_batch; NULL: none)
 };
 struct ListBlocks {
     int block0[LIST_MAXJOB + 1];
@@ -87,6 +88,14 @@ void list_scan_batch(const ScanJob* jobs, int n, hipStream_t st);
 void list_fill_batch(const ListJob* jobs, const ListBlocks& blk, hipStream_t st);
 // per bra of every job (block0 over nb_items): work items of its kets
 void list_items_count_batch(const ListJob* jobs, const ListBlocks& blk, hipStream_t st);
+// Order in which the ERI kernels visit each list (jobs' q[w], ord[w]; the
+// blocks of list_fill_batch): quartets by descending primitive work
+// (primitive pairs of bra x ket, in half-octave buckets), so that the lanes
+// of a wave loop over about as many primitive quartets as one another.  A
+// counting sort: per-bucket counts, their scan, then each quartet's slot.
+// cnt and cur: ORD_NB ints per list (2 per job); cnt zeroed here.
+#define ORD_NB 32
+void list_order_batch(const ListJob* jobs, const ListBlocks& blk, int* cnt, int* cur, hipStream_t st);
 // every work item of every job (block0 over nitems)
 void list_items_fill_batch(const ListJob* jobs, const ListBlocks& blk, hipStream_t st);
 

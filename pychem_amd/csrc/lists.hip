@@ -186,6 +186,63 @@ __global__ void __launch_bounds__(256) list_fill_batch_kernel(const ListJob* job
     if (q < J.nq[w]) list_fill_one(J.a, w ? J.b32 : J.b64, q, J.q[w], J.k[w]);
 }
 
+// Bucket of quartet i of a list: 0 holds the most primitive quartets (bra
+// pairs x ket pairs), half an octave per bucket.
+__device__ __forceinline__ int order_bucket(const ListJob& J, int w, long long i)
+{
+    const int2 pr = J.q[w][i];
+    const int work = max(1, J.a.pairs[pr.x].n * J.a.pairs[pr.y].n);
+    const int lg = 31 - __clz(work);                                  // floor(log2 work)
+    const int k = 2 * lg + (lg > 0 ? (work >> (lg - 1)) & 1 : 0);   // half-octaves
+    return ORD_NB - 1 - min(k, ORD_NB - 1);
+}
+
+__global__ void __launch_bounds__(256) list_order_count_kernel(const ListJob* jobs, const ListBlocks blk, int* cnt)
+{
+    __shared__ int h[ORD_NB];
+    int q;
+    const int jw = list_job_of(blk, q), w = jw & 1;
+    const ListJob& J = jobs[jw >> 1];
+    if (threadIdx.x < ORD_NB) h[threadIdx.x] = 0;
+    __syncthreads();
+    if (q < J.nq[w] && J.ord[w]) atomicAdd(&h[order_bucket(J, w, q)], 1);
+    __syncthreads();
+    if (threadIdx.x < ORD_NB && h[threadIdx.x]) atomicAdd(&cnt[jw * ORD_NB + threadIdx.x], h[threadIdx.x]);
+}
+
+// cur[list][bucket] = exclusive scan of the list's bucket counts
+__global__ void __launch_bounds__(64) list_order_scan_kernel(const int* cnt, int* cur, int nlist)
+{
+    for (int j = threadIdx.x; j < nlist; j += blockDim.x) {
+        int s = 0;
+        for (int k = 0; k < ORD_NB; ++k) {
+            cur[j * ORD_NB + k] = s;
+            s += cnt[j * ORD_NB + k];
+        }
+    }
+}
+
+// Every quartet takes a slot in its bucket: one global atomic per block and
+// bucket reserves the block's range, an LDS atomic gives the place in it.
+__global__ void __launch_bounds__(256) list_order_fill_kernel(const ListJob* jobs, const ListBlocks blk, int* cur)
+{
+    __shared__ int h[ORD_NB], base[ORD_NB];
+    int q;
+    const int jw = list_job_of(blk, q), w = jw & 1;
+    const ListJob& J = jobs[jw >> 1];
+    if (threadIdx.x < ORD_NB) h[threadIdx.x] = 0;
+    __syncthreads();
+    int bk = -1, slot = 0;
+    if (q < J.nq[w] && J.ord[w]) {
+        bk = order_bucket(J, w, q);
+        slot = atomicAdd(&h[bk], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < ORD_NB && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cur[jw * ORD_NB + threadIdx.x], h[threadIdx.x]);
+    __syncthreads();
+    if (bk >= 0) J.ord[w][base[bk] + slot] = q;
+}
+
 __global__ void __launch_bounds__(256) list_items_count_batch_kernel(const ListJob* jobs, const ListBlocks blk)
 {
     int i;
@@ -348,6 +405,15 @@ void list_fill_batch(const ListJob* jobs, const ListBlocks& blk, hipStream_t st)
 {
     if (blk.n > 0 && blk.block0[blk.n] > 0)
         hipLaunchKernelGGL(list_fill_batch_kernel, dim3(blk.block0[blk.n]), dim3(256), 0, st, jobs, blk);
+}
+
+void list_order_batch(const ListJob* jobs, const ListBlocks& blk, int* cnt, int* cur, hipStream_t st)
+{
+    if (blk.n <= 0 || blk.block0[blk.n] <= 0) return;
+    hipMemsetAsync(cnt, 0, sizeof(int) * ORD_NB * blk.n, st);
+    hipLaunchKernelGGL(list_order_count_kernel, dim3(blk.block0[blk.n]), dim3(256), 0, st, jobs, blk, cnt);
+    hipLaunchKernelGGL(list_order_scan_kernel, dim3(1), dim3(64), 0, st, (const int*)cnt, cur, blk.n);
+    hipLaunchKernelGGL(list_order_fill_kernel, dim3(blk.block0[blk.n]), dim3(256), 0, st, jobs, blk, cur);
 }
 
 void list_items_count_batch(const ListJob* jobs, const ListBlocks& blk, hipStream_t st)

@@ -166,3 +166,39 @@ def test_level_b_surface_matches_oracle(gpu):
     assert abs(integrals.nuclear_repulsion(m) - sum(
         ai.NuclearCharge * aj.NuclearCharge / np.linalg.norm(np.subtract(ai.Coordinates, aj.Coordinates))
         for i, ai in enumerate(m.Atoms) for aj in m.Atoms[:i])) < 1e-12
+
+
+@pytest.mark.parametrize("fp32_threshold", [0.0, 1e-6], ids=["fp64", "mixed"])
+def test_eri_values_do_not_depend_on_quartet_order(gpu, tmp_path, fp32_threshold):
+    """The ERI kernels visit the quartets of narrow classes by descending
+    primitive work; each quartet's arithmetic is unchanged, so the stored
+    values equal those computed in list order (PQ_ERI_ORDER=0, in a child
+    process since the switch is read once per process) bit for bit."""
+    import os
+    import subprocess
+    import sys
+    import textwrap
+    from pychem_amd.engine import Engine
+    rows = geometry.small_molecule("C2H6", 0.05, 5)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = str(tmp_path / "eri_list_order.npy")
+    code = textwrap.dedent("""
+        import sys, numpy as np
+        sys.path.insert(0, %r)
+        from pychem_amd import geometry, structures
+        from pychem_amd.engine import Engine
+        m = structures.Molecule(geometry.small_molecule("C2H6", 0.05, 5), basis="6-31G*", cartesian_l=[2])
+        e = Engine(m)
+        e.build_eri(1e-8, fp32_threshold=%r)
+        np.save(%r, e.eri_dense().cpu().numpy())
+    """) % (root, fp32_threshold, out)
+    env = dict(os.environ, PQ_ERI_ORDER="0")
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0, r.stderr[-2000:]
+    e = Engine(_mol(rows, "6-31G*"))
+    e.build_eri(1e-8, fp32_threshold=fp32_threshold)
+    g = e.eri_dense().cpu().numpy()
+    ref = np.load(out)
+    assert g.shape == ref.shape
+    assert np.array_equal(g, ref)
+    assert np.abs(g).max() > 1.0
