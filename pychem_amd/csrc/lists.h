@@ -93,8 +93,10 @@ void list_items_count_batch(const ListJob* jobs, const ListBlocks& blk, hipStrea
 // (primitive pairs of bra x ket, in half-octave buckets), so that the lanes
 // of a wave loop over about as many primitive quartets as one another.  A
 // counting sort: per-bucket counts, their scan, then each quartet's slot.
-// cnt and cur: ORD_NB ints per list (2 per job); cnt zeroed here.
+// cnt and cur: ORD_NB ints per list (2 per job); cnt zeroed here.  blk:
+// ceil(nq / (256 ORD_PT)) blocks per list, none for lists without an order.
 #define ORD_NB 32
+#define ORD_PT 16
 void list_order_batch(const ListJob* jobs, const ListBlocks& blk, int* cnt, int* cur, hipStream_t st);
 // every work item of every job (block0 over nitems)
 void list_items_fill_batch(const ListJob* jobs, const ListBlocks& blk, hipStream_t st);

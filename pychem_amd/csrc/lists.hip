@@ -197,15 +197,31 @@ __device__ __forceinline__ int order_bucket(const ListJob& J, int w, long long i
     return ORD_NB - 1 - min(k, ORD_NB - 1);
 }
 
+// list of block blockIdx.x (blocks of ORD_PT * 256 quartets) and the
+// block's index within it
+__device__ __forceinline__ int order_job_of(const ListBlocks& blk, int& lb)
+{
+    const int b = blockIdx.x;
+    int j = 0;
+    while (j + 1 < blk.n && blk.block0[j + 1] <= b) ++j;
+    lb = b - blk.block0[j];
+    return j;
+}
+
 __global__ void __launch_bounds__(256) list_order_count_kernel(const ListJob* jobs, const ListBlocks blk, int* cnt)
 {
     __shared__ int h[ORD_NB];
-    int q;
-    const int jw = list_job_of(blk, q), w = jw & 1;
+    int lb;
+    const int jw = order_job_of(blk, lb), w = jw & 1;
     const ListJob& J = jobs[jw >> 1];
     if (threadIdx.x < ORD_NB) h[threadIdx.x] = 0;
     __syncthreads();
-    if (q < J.nq[w] && J.ord[w]) atomicAdd(&h[order_bucket(J, w, q)], 1);
+    const long long q0 = (long long)lb * 256 * ORD_PT + threadIdx.x;
+    if (J.ord[w])
+        for (int u = 0; u < ORD_PT; ++u) {
+            const long long q = q0 + 256 * u;
+            if (q < J.nq[w]) atomicAdd(&h[order_bucket(J, w, q)], 1);
+        }
     __syncthreads();
     if (threadIdx.x < ORD_NB && h[threadIdx.x]) atomicAdd(&cnt[jw * ORD_NB + threadIdx.x], h[threadIdx.x]);
 }
@@ -227,20 +243,29 @@ __global__ void __launch_bounds__(64) list_order_scan_kernel(const int* cnt, int
 __global__ void __launch_bounds__(256) list_order_fill_kernel(const ListJob* jobs, const ListBlocks blk, int* cur)
 {
     __shared__ int h[ORD_NB], base[ORD_NB];
-    int q;
-    const int jw = list_job_of(blk, q), w = jw & 1;
+    int lb;
+    const int jw = order_job_of(blk, lb), w = jw & 1;
     const ListJob& J = jobs[jw >> 1];
     if (threadIdx.x < ORD_NB) h[threadIdx.x] = 0;
     __syncthreads();
-    int bk = -1, slot = 0;
-    if (q < J.nq[w] && J.ord[w]) {
-        bk = order_bucket(J, w, q);
-        slot = atomicAdd(&h[bk], 1);
+    const long long q0 = (long long)lb * 256 * ORD_PT + threadIdx.x;
+    int bk[ORD_PT], slot[ORD_PT];
+#pragma unroll
+    for (int u = 0; u < ORD_PT; ++u) {
+        const long long q = q0 + 256 * u;
+        bk[u] = -1;
+        slot[u] = 0;
+        if (J.ord[w] && q < J.nq[w]) {
+            bk[u] = order_bucket(J, w, q);
+            slot[u] = atomicAdd(&h[bk[u]], 1);
+        }
     }
     __syncthreads();
     if (threadIdx.x < ORD_NB && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cur[jw * ORD_NB + threadIdx.x], h[threadIdx.x]);
     __syncthreads();
-    if (bk >= 0) J.ord[w][base[bk] + slot] = q;
+#pragma unroll
+    for (int u = 0; u < ORD_PT; ++u)
+        if (bk[u] >= 0) J.ord[w][base[bk[u]] + slot[u]] = (int)(q0 + 256 * u);
 }
 
 __global__ void __launch_bounds__(256) list_items_count_batch_kernel(const ListJob* jobs, const ListBlocks blk)

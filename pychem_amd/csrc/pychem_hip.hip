@@ -354,7 +354,15 @@ struct pq_basis {
     int have_bound = 0;
     // per pair class: pair ids sorted by (molecule, bound desc), molecule
     // segment starts, cumulative primitive-pair counts (device)
-    DevBuf d_sorted[6], d_seg[6], d_npcum[6], d_sbound[6], d_total;
+    // per pair class: sorted pair ids, molecule segments, cumulative
+    // primitive-pair counts and sorted bounds, all in one device buffer
+    // (d_sortpack, one upload of hsortpack per plan)
+    DevBuf d_sortpack, d_total;
+    const int* p_sorted[6] = {};
+    const int* p_seg[6] = {};
+    const long long* p_npcum[6] = {};
+    const double* p_sbound[6] = {};
+    std::vector<char> hsortpack;
     std::vector<int> hseg[6], hsorted[6];    // host copies of what the device lists read
     std::vector<double> hsbound[6];
     std::vector<long long> hnpcum[6];
@@ -633,9 +641,7 @@ extern "C" void pq_basis_destroy(pq_basis* b)
             t.qs.release(); t.nit.release(); t.its.release();
         }
     }
-    for (int X = 0; X < 6; ++X) {
-        b->d_sorted[X].release(); b->d_seg[X].release(); b->d_npcum[X].release(); b->d_sbound[X].release();
-    }
+    b->d_sortpack.release();
     b->d_pairs.release(); b->d_prims.release(); b->d_atoms.release();
     b->d_cart_off.release(); b->d_bound.release(); b->d_tmp_list.release(); b->d_spher.release();
     b->d_mols.release(); b->d_total.release(); b->d_jred.release();
@@ -882,26 +888,43 @@ static int upload_sorted_pairs(pq_basis* b, hipStream_t st)
         for (size_t k = 0; k < sorted.size(); ++k) npcum[k + 1] = npcum[k] + b->pairs[sorted[k]].n;
         std::vector<double> sb(std::max<size_t>(1, sorted.size()), 0.0);
         for (size_t k = 0; k < sorted.size(); ++k) sb[k] = b->bound[sorted[k]];
-        const int n = (int)sorted.size();
-        b->nsorted[X] = n;
-        if (b->d_sorted[X].ensure(sizeof(int) * std::max(1, n)) || b->d_seg[X].ensure(sizeof(int) * seg.size()) ||
-            b->d_npcum[X].ensure(sizeof(long long) * npcum.size()) || b->d_sbound[X].ensure(sizeof(double) * sb.size()))
-            return -1;
-        // the host copies live in the handle until the next plan, so the
-        // asynchronous uploads need no wait here
+        b->nsorted[X] = (int)sorted.size();
         b->hsorted[X] = std::move(sorted);
         b->hnpcum[X] = std::move(npcum);
         b->hseg[X] = std::move(seg);
         b->hsbound[X] = std::move(sb);
-        if (n)
-            HIPCHK(hipMemcpyAsync(b->d_sorted[X].p, b->hsorted[X].data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(b->d_seg[X].p, b->hseg[X].data(), sizeof(int) * b->hseg[X].size(), hipMemcpyHostToDevice,
-                              st));
-        HIPCHK(hipMemcpyAsync(b->d_npcum[X].p, b->hnpcum[X].data(), sizeof(long long) * b->hnpcum[X].size(),
-                              hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(b->d_sbound[X].p, b->hsbound[X].data(), sizeof(double) * b->hsbound[X].size(),
-                              hipMemcpyHostToDevice, st));
     }
+    // one upload of every class's arrays (separate copies from pageable
+    // memory cost ~20 us of host time each, 24 per plan); 256-byte sections
+    size_t off[6][4], total = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = total;
+        total += (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+        return o;
+    };
+    for (int X = 0; X < 6; ++X) {
+        off[X][0] = take(sizeof(int) * b->hsorted[X].size());
+        off[X][1] = take(sizeof(int) * b->hseg[X].size());
+        off[X][2] = take(sizeof(long long) * b->hnpcum[X].size());
+        off[X][3] = take(sizeof(double) * b->hsbound[X].size());
+    }
+    if (b->d_sortpack.ensure(total)) return -1;
+    // the host pack lives in the handle until the next plan, so the
+    // asynchronous upload needs no wait here
+    b->hsortpack.assign(total, 0);
+    char* h = b->hsortpack.data();
+    char* d = (char*)b->d_sortpack.p;
+    for (int X = 0; X < 6; ++X) {
+        memcpy(h + off[X][0], b->hsorted[X].data(), sizeof(int) * b->hsorted[X].size());
+        memcpy(h + off[X][1], b->hseg[X].data(), sizeof(int) * b->hseg[X].size());
+        memcpy(h + off[X][2], b->hnpcum[X].data(), sizeof(long long) * b->hnpcum[X].size());
+        memcpy(h + off[X][3], b->hsbound[X].data(), sizeof(double) * b->hsbound[X].size());
+        b->p_sorted[X] = (const int*)(d + off[X][0]);
+        b->p_seg[X] = (const int*)(d + off[X][1]);
+        b->p_npcum[X] = (const long long*)(d + off[X][2]);
+        b->p_sbound[X] = (const double*)(d + off[X][3]);
+    }
+    HIPCHK(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, st));
     return 0;
 }
 
@@ -909,13 +932,13 @@ static ListArgs list_args(const pq_basis* b, int c)
 {
     const int X = eri_class_table[c][5], Y = eri_class_table[c][6];
     ListArgs a;
-    a.bra = (const int*)b->d_sorted[X].p;
-    a.ket = (const int*)b->d_sorted[Y].p;
-    a.bra_seg = (const int*)b->d_seg[X].p;
-    a.ket_seg = (const int*)b->d_seg[Y].p;
-    a.ket_npcum = (const long long*)b->d_npcum[Y].p;
+    a.bra = b->p_sorted[X];
+    a.ket = b->p_sorted[Y];
+    a.bra_seg = b->p_seg[X];
+    a.ket_seg = b->p_seg[Y];
+    a.ket_npcum = b->p_npcum[Y];
     a.bound = (const double*)b->d_bound.p;
-    a.bra_bound = (const double*)b->d_sbound[X].p;
+    a.bra_bound = b->p_sbound[X];
     a.pairs = (const PairHdr*)b->d_pairs.p;
     a.cart_off = (const int*)b->d_cart_off.p;
     a.thr = b->threshold;
@@ -1087,7 +1110,8 @@ extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold,
         std::vector<ListJob> next = b->hjobs.back();   // the next phase's table: a new host copy
         b->hjobs.push_back(std::move(next));
     }
-    ListBlocks fill_blocks = {};
+    ListBlocks order_blocks = {};
+    order_blocks.n = 2 * ERI_NCLASS;
     {
         std::vector<ListJob>& jobs = b->hjobs.back();
         ListBlocks fb = {};
@@ -1118,11 +1142,13 @@ extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold,
                     jobs[c].ord[w] = s.ncomp <= ORDER_MAX_NCOMP ? (int*)s.order.p : nullptr;
                 }
                 fb.block0[2 * c + w + 1] = fb.block0[2 * c + w] + (int)((nq + 255) / 256);
+                order_blocks.block0[2 * c + w + 1] =
+                    order_blocks.block0[2 * c + w] +
+                    (jobs[c].ord[w] ? (int)((nq + 256 * ORD_PT - 1) / (256 * ORD_PT)) : 0);
             }
         }
         if (upload_jobs(jobs)) return -1;
         list_fill_batch((const ListJob*)b->d_jobs.p, fb, st);
-        fill_blocks = fb;
     }
     {
         std::vector<ListJob> next = b->hjobs.back();   // the next phase's table: a new host copy
@@ -1189,10 +1215,10 @@ extern "C" int pq_eri_plan(pq_basis* b, double threshold, double fp32_threshold,
         }();
         b->eri_order = order_env != 0;
         if (b->eri_order) {
-            const ListBlocks& fb = fill_blocks;
-            if (b->d_order_cnt.ensure(sizeof(int) * 2 * ORD_NB * fb.n)) return -1;
+            const ListBlocks& ob = order_blocks;
+            if (b->d_order_cnt.ensure(sizeof(int) * 2 * ORD_NB * ob.n)) return -1;
             int* cnt = (int*)b->d_order_cnt.p;
-            list_order_batch((const ListJob*)b->d_jobs.p, fb, cnt, cnt + ORD_NB * fb.n, st);
+            list_order_batch((const ListJob*)b->d_jobs.p, ob, cnt, cnt + ORD_NB * ob.n, st);
         }
     }
     HIPCHK(hipGetLastError());
