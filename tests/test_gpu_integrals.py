@@ -45,9 +45,11 @@ def test_one_electron(gpu, case):
     assert np.abs(V - Vo).max() < 1e-10
 
 
-def test_schwarz(gpu):
+@pytest.mark.parametrize("cart", [(2,), ()], ids=["6d", "5d"])
+def test_schwarz(gpu, cart):
+    """Schwarz bounds vs the oracle, Cartesian and spherical d shells."""
     from pychem_amd.engine import Engine
-    m = _mol(geometry.alkane(3, 0.02, 3003), "6-31G*")
+    m = _mol(geometry.alkane(3, 0.02, 3003), "6-31G*", cart=cart)
     e = Engine(m)
     g = e.schwarz()
     o = cint.schwarz(m)
