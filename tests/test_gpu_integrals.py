@@ -1,6 +1,6 @@
 """GPU parity: HIP engine vs the CPU oracle (oracle/oracle.c) on the same inputs.
 
-Bar: integrals and J/K to 1e-10 absolute (north-star parity), Boys to 1e-13
+Bar: integrals and J/K to 1e-10 absolute (the parity target), Boys to 1e-13
 relative.
 """
 import numpy as np
